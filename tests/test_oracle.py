@@ -1,0 +1,380 @@
+"""Pins for the CPU oracle (oracle/ppf_oracle.c).
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped term,
+a wrong sign or index, a transposed operand) fails it, and none of them retypes
+the oracle's own formula: transforms against scipy/numpy, eigenvalues against a
+dense eigen-decomposition, Thomas against the reference's own tdmas.cpp compiled
+from source, the whole solve against dense linear algebra on tiny grids and
+against discrete eigen-mode manufactured solutions.
+"""
+import itertools
+
+import numpy as np
+import pytest
+import scipy.fft as sf
+
+from oracle import oracle as O
+from pascal_poisson_fft_amd.grid import make_axis, cell_centres
+from pascal_poisson_fft_amd import workloads as W
+
+SIZES = [2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 512]
+
+
+# ----------------------------------------------------------------- grid (A18)
+@pytest.mark.parametrize("nm,gamma", [(8, 1.5), (64, 1.5), (96, 1.8), (33, 1.0)])
+def test_grid_tanh_invariants(nm, gamma):
+    x, dx, dmx = O.grid(False, nm, 2.0, gamma, False)
+    assert x[1] == 0.0 and abs(x[nm + 1] - 2.0) < 1e-15
+    assert x[0] == x[1]  # wall ghost node on the wall
+    assert np.all(np.diff(x[1:]) > 0)
+    # tanh grid is symmetric about the channel centre
+    np.testing.assert_allclose(x[1:nm + 2] + x[1:nm + 2][::-1], 2.0, rtol=0, atol=1e-14)
+    assert abs(dx[1:nm + 1].sum() - 2.0) < 1e-14
+    assert dx[0] == 0.0 and dx[nm + 1] == 0.0
+    # centre spacing = distance between cell centres; half cells at the walls
+    c = 0.5 * (x[1:nm + 1] + x[2:nm + 2])
+    np.testing.assert_allclose(dmx[2:nm + 1], np.diff(c), rtol=1e-13)
+    assert abs(dmx[1] - 0.5 * dx[1]) < 1e-16 and abs(dmx[nm + 1] - 0.5 * dx[nm]) < 1e-16
+
+
+def test_grid_uniform_and_periodic():
+    x, dx, dmx = O.grid(True, 16, 2 * np.pi, 0.0, True)
+    h = 2 * np.pi / 16
+    np.testing.assert_allclose(dx, h, rtol=1e-14)
+    np.testing.assert_allclose(dmx, h, rtol=1e-14)
+    assert abs(x[0] + h) < 1e-15  # periodic: ghost node one cell below
+
+
+@pytest.mark.parametrize("uniform,pbc", [(True, True), (True, False), (False, False)])
+def test_product_grid_matches_oracle(uniform, pbc):
+    for nm in (8, 48, 512):
+        a = make_axis(nm, 2.0, pbc, uniform=uniform, gamma=1.5)
+        x, dx, dmx = O.grid(uniform, nm, 2.0, 1.5, pbc)
+        np.testing.assert_allclose(a.x, x, rtol=0, atol=1e-15)  # libm vs numpy tanh: an ulp
+        np.testing.assert_allclose(a.dx, dx, rtol=0, atol=2e-15)
+        np.testing.assert_allclose(a.dmx, dmx, rtol=0, atol=2e-15)
+
+
+# ------------------------------------------------------------ eigenvalues (A4)
+def _second_difference(n, periodic):
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, i] = -2.0
+        if periodic:
+            A[i, (i - 1) % n] += 1.0
+            A[i, (i + 1) % n] += 1.0
+        else:
+            if i > 0:
+                A[i, i - 1] = 1.0
+            else:
+                A[i, i] += 1.0  # zero-flux wall: mirror ghost
+            if i < n - 1:
+                A[i, i + 1] = 1.0
+            else:
+                A[i, i] += 1.0
+    return A
+
+
+@pytest.mark.parametrize("pbc1,pbc2", list(itertools.product([False, True], repeat=2)))
+def test_eigenvalues_diagonalise_the_difference_operators(pbc1, pbc2):
+    n1, n2, L1, L2 = 12, 16, 2.0, 3.0
+    dxk2, dyk2 = O.eigen(pbc1, pbc2, n1, n2, L1, L2)
+    for n, L, per, lam, axis in ((n1, L1, pbc1, dxk2, 0), (n2, L2, pbc2, dyk2, 1)):
+        A = _second_difference(n, per) / (L / n) ** 2
+        i = np.arange(n)
+        for m in range(n):
+            if per:
+                mm = m
+                if axis == 1 and pbc1 and pbc2:  # P-P: entry m of dyk2 is the wrapped mode
+                    mm = m if m <= n // 2 else m - n
+                v = np.exp(2j * np.pi * mm * i / n)
+            else:
+                v = np.cos(np.pi * m * (2 * i + 1) / (2 * n))
+            np.testing.assert_allclose(A @ v, -lam[m] * v, atol=1e-10 * max(1.0, lam.max()))
+
+
+# -------------------------------------------------------------- transforms (A5-A9)
+@pytest.mark.parametrize("n", SIZES)
+def test_dct2_matches_scipy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n)
+    X = O.dct2(x)
+    ref = sf.dct(x, type=2)
+    np.testing.assert_allclose(X, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+    np.testing.assert_allclose(O.idct2(X), x, rtol=0, atol=1e-13 * np.abs(x).max())
+    np.testing.assert_allclose(O.idct2(ref), sf.idct(ref, type=2), rtol=0, atol=1e-13 * np.abs(x).max())
+
+
+def _reference_reorder_fwd(x):
+    """DCT_f_pre exactly as indexed at fft_3d_poisson.f90:924-933 (1-based)."""
+    n = x.size
+    out = np.full(n, np.nan)
+    for i in range(1, n // 2 + 1):
+        out[i - 1] = x[(2 * i - 1) - 1]
+        out[i + n // 2 - 1] = x[(2 * (n - (i + (n // 2)) + 1)) - 1]
+    return out
+
+
+def _reference_reorder_inv(v):
+    """DCT_b_post exactly as indexed at fft_3d_poisson.f90:1022-1031 (1-based)."""
+    n = v.size
+    out = np.full(n, np.nan)
+    for i in range(1, n // 2 + 1):
+        out[(2 * i - 1) - 1] = v[i - 1]
+        out[(2 * (n - (i + (n // 2)) + 1)) - 1] = v[i + n // 2 - 1]
+    return out
+
+
+@pytest.mark.parametrize("n", [8, 16, 48, 512])
+def test_reference_makhoul_indexing_is_scipy_dct2(n):
+    """The reference's reorder + R2C + post-twiddle (:924-969) is scipy's DCT-II,
+    and its inverse reorder (:1022-1031) undoes the forward one exactly; the
+    oracle's DCT equals that composition."""
+    x = np.random.default_rng(n).standard_normal(n)
+    v = _reference_reorder_fwd(x)
+    assert not np.isnan(v).any()
+    V = np.fft.rfft(v)
+    X = np.empty(n)
+    k = np.arange(n // 2)
+    arg = -np.pi * k / (2.0 * n)
+    X[: n // 2] = 2.0 * (np.cos(arg) * V[k].real - np.sin(arg) * V[k].imag)
+    k2 = np.arange(n // 2, n)
+    arg2 = -np.pi * k2 / (2.0 * n)
+    X[n // 2:] = 2.0 * (np.cos(arg2) * V[n - k2].real + np.sin(arg2) * V[n - k2].imag)
+    np.testing.assert_allclose(X, sf.dct(x, type=2), atol=1e-13 * np.abs(X).max())
+    np.testing.assert_array_equal(_reference_reorder_inv(v), x)
+    np.testing.assert_allclose(O.dct2(x), X, atol=1e-13 * np.abs(X).max())
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_rfft_irfft_match_numpy(n):
+    rng = np.random.default_rng(100 + n)
+    x = rng.standard_normal(n)
+    X = O.rfft(x)
+    np.testing.assert_allclose(X, np.fft.rfft(x), rtol=0, atol=1e-13 * np.abs(X).max())
+    # C2R ignores the imaginary parts of the DC and Nyquist terms (unnormalised)
+    Y = X.copy()
+    Y[0] += 0.7j
+    Y[-1] += -0.3j
+    np.testing.assert_allclose(O.irfft(Y, n), np.fft.irfft(X, n) * n, atol=1e-12 * n)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 6, 7, 12, 24, 48, 96, 100, 192, 256])
+def test_complex_fft_matches_numpy(n):
+    rng = np.random.default_rng(7 * n)
+    z = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    np.testing.assert_allclose(O.fft(z, -1), np.fft.fft(z), atol=1e-12 * max(1, n))
+    np.testing.assert_allclose(O.fft(z, +1), np.fft.ifft(z) * n, atol=1e-12 * max(1, n))
+
+
+# ----------------------------------------------------------------- Thomas (A13)
+def _random_dd_system(nrow, nsys, seed):
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.1, 1.0, (nrow, nsys))
+    c = rng.uniform(0.1, 1.0, (nrow, nsys))
+    b = -(a + c) - rng.uniform(0.0, 2.0, (nrow, nsys))
+    a[0] = 0.0
+    c[-1] = 0.0
+    d = rng.standard_normal((nrow, nsys))
+    return a, b, c, d
+
+
+@pytest.mark.skipif(not O.ref_tdma_available(), reason="reference tdmas.cpp not built (no reference checkout)")
+@pytest.mark.parametrize("nrow,nsys", [(2, 1), (7, 5), (64, 33), (512, 8)])
+def test_thomas_bitwise_vs_reference_tdmas(nrow, nsys):
+    a, b, c, d = _random_dd_system(nrow, nsys, nrow * 31 + nsys)
+    mine = O.tdma_many(a, b, c, d)
+    ref = O.ref_tdma_many(a, b, c, d)
+    assert np.array_equal(mine, ref)
+
+
+def test_thomas_vs_dense_solve():
+    nrow, nsys = 40, 6
+    a, b, c, d = _random_dd_system(nrow, nsys, 5)
+    x = O.tdma_many(a, b, c, d)
+    for s in range(nsys):
+        A = np.diag(b[:, s]) + np.diag(a[1:, s], -1) + np.diag(c[:-1, s], 1)
+        np.testing.assert_allclose(x[:, s], np.linalg.solve(A, d[:, s]), rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------------ whole solve (A1, A11-A17)
+def _dense_operator(ax, ay, az):
+    """Assemble the 7-point operator cell by cell from its finite-volume definition."""
+    n1, n2, n3 = ax.nm, ay.nm, az.nm
+    N = n1 * n2 * n3
+    A = np.zeros((N, N))
+
+    def q(i, j, k):
+        return (k * n2 + j) * n1 + i
+
+    for k in range(n3):
+        for j in range(n2):
+            for i in range(n1):
+                r = q(i, j, k)
+                for (a, idx, n) in ((ax, i, n1), (ay, j, n2)):
+                    h2 = (a.L / n) ** 2
+                    for step in (-1, 1):
+                        t = idx + step
+                        if a.pbc:
+                            t %= n
+                        elif t < 0 or t >= n:
+                            continue  # zero flux through the wall
+                        ii, jj = (t, j) if a is ax else (i, t)
+                        A[r, q(ii, jj, k)] += 1.0 / h2
+                        A[r, r] -= 1.0 / h2
+                # z: flux form with face spacing dmx and cell width dx (1-based metrics)
+                K = k + 1
+                if k > 0:
+                    w = 1.0 / (az.dx[K] * az.dmx[K])
+                    A[r, q(i, j, k - 1)] += w
+                    A[r, r] -= w
+                if k < n3 - 1:
+                    w = 1.0 / (az.dx[K] * az.dmx[K + 1])
+                    A[r, q(i, j, k + 1)] += w
+                    A[r, r] -= w
+    return A
+
+
+def _brute_force(f, ax, ay, az):
+    """Dense solve of the pinned problem: the pin at the (0,0) mode's first row
+    (fft_3d_poisson.f90:1152-1157) makes the solve satisfy L p = f - delta on
+    the first z-plane, delta = sum_ijk dx3_k f_ijk / (n1 n2 dx3_1); the mean is
+    then removed (:628-664)."""
+    n3, n2, n1 = f.shape
+    dz = az.dx[1:az.nm + 1]
+    delta = float(np.einsum("kji,k->", f, dz)) / (n1 * n2 * dz[0])
+    g = f.copy()
+    g[0] -= delta
+    A = _dense_operator(ax, ay, az)
+    M = np.vstack([A, np.ones((1, A.shape[1]))])
+    rhs = np.concatenate([g.ravel(), [0.0]])
+    p, *_ = np.linalg.lstsq(M, rhs, rcond=None)
+    return p.reshape(f.shape)
+
+
+CASES = [
+    (8, 6, 10, (False, False), False, 1.5),
+    (8, 6, 10, (True, True), True, 0.0),
+    (12, 8, 6, (True, False), False, 2.0),
+    (6, 12, 8, (False, True), False, 1.0),
+    (8, 8, 6, (False, False), True, 0.0),
+    (12, 6, 8, (True, True), False, 1.8),
+]
+
+
+@pytest.mark.parametrize("n1,n2,n3,pbc,uni,gamma", CASES)
+@pytest.mark.parametrize("compatible", [True, False])
+def test_solve_vs_dense_brute_force(n1, n2, n3, pbc, uni, gamma, compatible):
+    ax = make_axis(n1, 2.0, pbc[0])
+    ay = make_axis(n2, 3.0, pbc[1])
+    az = make_axis(n3, 1.5, False, uniform=uni, gamma=gamma)
+    rng = np.random.default_rng(n1 * 100 + n2 * 10 + n3)
+    f = rng.standard_normal((n3, n2, n1))
+    if compatible:
+        f = W.remove_weighted_mean(f, az)
+    P = O.solve(f, (ax.L, ay.L, az.L), (pbc[0], pbc[1], False), ax.dmx, ay.dmx, az.dmx, az.dx)
+    ref = _brute_force(f, ax, ay, az)
+    np.testing.assert_allclose(P[1:-1, 1:-1, 1:-1], ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+
+
+def _wall_extrapolate(p1, p2, d1, d2):
+    """Value at the wall of the even quadratic A + B s^2 (zero wall slope)
+    through p1 at distance d1 and p2 at distance d1 + d2 from the wall."""
+    s1, s2 = d1 ** 2, (d1 + d2) ** 2
+    return (p1 * s2 - p2 * s1) / (s2 - s1)
+
+
+@pytest.mark.parametrize("pbc", [(False, False), (True, False), (False, True), (True, True)])
+def test_solve_ghost_layers(pbc):
+    n1, n2, n3 = 8, 12, 10
+    ax = make_axis(n1, 2.0, pbc[0])
+    ay = make_axis(n2, 2.0, pbc[1])
+    az = make_axis(n3, 2.0, False, uniform=False, gamma=1.5)
+    f = W.remove_weighted_mean(np.random.default_rng(3).standard_normal((n3, n2, n1)), az)
+    P = O.solve(f, (2.0, 2.0, 2.0), (pbc[0], pbc[1], False), ax.dmx, ay.dmx, az.dmx, az.dx)
+    # tensor-product ghost rule: each axis' ghosts from that axis' neighbours,
+    # built from the interior only (independent of the reference's sweep order)
+    G = P[1:-1, 1:-1, 1:-1].copy()
+    for axis, a in ((2, ax), (1, ay), (0, az)):
+        n = a.nm
+        lo_centre = a.dmx[1]  # wall to first centre
+        if a.pbc:
+            lo = G.take([n - 1], axis=axis)
+            hi = G.take([0], axis=axis)
+        else:
+            zc = cell_centres(a)
+            d1, d2 = zc[0] - a.x[1], zc[1] - zc[0]
+            lo = _wall_extrapolate(G.take([0], axis=axis), G.take([1], axis=axis), d1, d2)
+            e1, e2 = a.x[n + 1] - zc[-1], zc[-1] - zc[-2]
+            hi = _wall_extrapolate(G.take([n - 1], axis=axis), G.take([n - 2], axis=axis), e1, e2)
+            assert abs(lo_centre - d1) < 1e-15
+        G = np.concatenate([lo, G, hi], axis=axis)
+    np.testing.assert_allclose(P, G, rtol=0, atol=1e-12 * np.abs(G).max())
+    # the solution has zero (unweighted) interior mean
+    assert abs(P[1:-1, 1:-1, 1:-1].mean()) < 1e-14 * np.abs(P).max()
+
+
+@pytest.mark.parametrize("pbc", [(False, False), (True, True), (True, False), (False, True)])
+def test_solve_recovers_discrete_eigenmode(pbc):
+    """u = phi_x(m1) phi_y(m2) w(z) with phi discrete eigenvectors of the x/y
+    operators: then L u = phi_x phi_y (T_z - lx - ly) w, solved exactly."""
+    n1, n2, n3 = 32, 24, 40
+    ax = make_axis(n1, 2.0, pbc[0])
+    ay = make_axis(n2, 3.0, pbc[1])
+    az = make_axis(n3, 2.0, False, uniform=False, gamma=1.7)
+    m1, m2 = 3, 5
+    i, j = np.arange(n1), np.arange(n2)
+    if pbc[0]:
+        px, lx = np.cos(2 * np.pi * m1 * i / n1), 2 * (1 - np.cos(2 * np.pi * m1 / n1)) / (ax.L / n1) ** 2
+    else:
+        px, lx = np.cos(np.pi * m1 * (2 * i + 1) / (2 * n1)), 4 * np.sin(np.pi * m1 / (2 * n1)) ** 2 / (ax.L / n1) ** 2
+    if pbc[1]:
+        py, ly = np.sin(2 * np.pi * m2 * j / n2), 2 * (1 - np.cos(2 * np.pi * m2 / n2)) / (ay.L / n2) ** 2
+    else:
+        py, ly = np.cos(np.pi * m2 * (2 * j + 1) / (2 * n2)), 4 * np.sin(np.pi * m2 / (2 * n2)) ** 2 / (ay.L / n2) ** 2
+    zc = cell_centres(az)
+    w = np.exp(-zc) * np.cos(2.3 * zc)
+    am, ap = W.z_rows(az)
+    tw = am * np.concatenate([[0.0], w[:-1]]) - (am + ap) * w + ap * np.concatenate([w[1:], [0.0]])
+    g = tw - (lx + ly) * w
+    f = g[:, None, None] * py[None, :, None] * px[None, None, :]
+    u = w[:, None, None] * py[None, :, None] * px[None, None, :]
+    P = O.solve(f, (ax.L, ay.L, az.L), (pbc[0], pbc[1], False), ax.dmx, ay.dmx, az.dmx, az.dx)
+    np.testing.assert_allclose(P[1:-1, 1:-1, 1:-1], u - u.mean(), rtol=0, atol=1e-11 * np.abs(u).max())
+
+
+def test_residual_invariant_and_planted_error():
+    prob = W.smoothed_normal((16, 12, 20), (False, False, False), 1.5, seed=9)
+    ax, ay, az = prob.ax
+    P = O.solve(prob.rhs, prob.L, prob.pbc, ax.dmx, ay.dmx, az.dmx, az.dx)
+    r = O.rel_residual(prob.rhs, P, prob.L, prob.pbc, az.dmx, az.dx)
+    assert r < 1e-12
+    # the host operator used to build workloads equals the oracle's
+    p = P[1:-1, 1:-1, 1:-1]
+    np.testing.assert_allclose(W.apply_operator(p, ax, ay, az), O.apply_L(p, prob.L, prob.pbc, az.dmx, az.dx),
+                               rtol=0, atol=1e-12 * np.abs(prob.rhs).max())
+    # a planted error in the solution must show up in the residual
+    Q = P.copy()
+    Q[5, 4, 3] += 1e-6 * np.abs(P).max()
+    assert O.rel_residual(prob.rhs, Q, prob.L, prob.pbc, az.dmx, az.dx) > 1e-9
+
+
+def test_c1_manufactured_accuracy():
+    prob = W.manufactured_c1(32)
+    ax, ay, az = prob.ax
+    P = O.solve(prob.rhs, prob.L, prob.pbc, ax.dmx, ay.dmx, az.dmx, az.dx)
+    err = np.abs(P[1:-1, 1:-1, 1:-1] - prob.exact).max()
+    # error is the noise response only (noise amplitude 1e-3, smooth inverse)
+    assert err < 1e-3
+    assert O.rel_residual(prob.rhs, P, prob.L, prob.pbc, az.dmx, az.dx) < 1e-12
+
+
+def test_solve_rejects_odd_and_periodic_z():
+    f = np.zeros((4, 4, 5))
+    ax, ay, az = make_axis(5, 1.0, False), make_axis(4, 1.0, False), make_axis(4, 1.0, False)
+    with pytest.raises(ValueError):
+        O.solve(f, (1, 1, 1), (False, False, False), ax.dmx, ay.dmx, az.dmx, az.dx)
+    ax = make_axis(4, 1.0, False)
+    f = np.zeros((4, 4, 4))
+    with pytest.raises(ValueError):
+        O.solve(f, (1, 1, 1), (False, False, True), ax.dmx, ay.dmx, az.dmx, az.dx)
