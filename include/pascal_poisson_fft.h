@@ -1,0 +1,133 @@
+/*
+ * pascal_poisson_fft.h -- C ABI of the MI355X (gfx950) Poisson solve path.
+ *
+ * Drop-in for the GPU solve path of PaScaL_POISSON_FFT (Fortran module
+ * `fft_poisson`, 01_Fortran/src/fft_3d_poisson.f90) and the one-rank branch of
+ * the batched Thomas library it calls (PaScaL_TDMA_cuda,
+ * 01_Fortran/PaScaL_TDMA/src/pascal_tdma_cuda.f90).  The Fortran module in
+ * pascal_poisson_fft_amd/fortran/fft_poisson.f90 binds these entry points with
+ * ISO_C_BINDING under the reference's own procedure names.
+ *
+ * Conventions
+ *   - every array is x-fastest (Fortran column-major), FP64, in device memory;
+ *   - all work is ordered on the given HIP stream (NULL = default stream) and
+ *     returns without synchronising, except where a function says otherwise;
+ *   - functions return PPF_OK (0) or a negative PPF_ERR_* code, never abort,
+ *     never throw; ppf_error_string() names a code;
+ *   - a plan may be used by one host thread at a time.
+ */
+#ifndef PASCAL_POISSON_FFT_H
+#define PASCAL_POISSON_FFT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PPF_OK 0
+#define PPF_ERR_INVALID_ARG (-1)   /* null pointer, negative size, bad enum */
+#define PPF_ERR_SIZE (-2)          /* an extent the transforms do not support */
+#define PPF_ERR_HIP (-3)           /* a HIP runtime call failed */
+#define PPF_ERR_ALLOC (-4)         /* device allocation failed */
+#define PPF_ERR_UNSUPPORTED (-5)   /* a configuration not implemented */
+#define PPF_ERR_NOT_CONVERGED (-6) /* refinement hit max_sweeps above tol */
+
+#define PPF_PREC_FP64 0    /* FP64 transforms and Thomas */
+#define PPF_PREC_FP32_IR 1 /* FP32 transforms/Thomas + FP64 residual refinement */
+
+typedef struct ppf_plan ppf_plan;
+
+/* Problem description.  Replaces the scalar arguments of
+ * fft_poisson_plan_cuda_create (fft_3d_poisson.f90:70-108): the reference
+ * passes n = nm+1 and derives nm = n-1 (:103-104); here the interior counts nm
+ * are given directly.  Extents must be even, 8 <= nm <= 1024 in x and y with
+ * nm of the form 2^a or 3*2^a, and nm >= 2 in z. */
+typedef struct ppf_desc {
+    int n1m, n2m, n3m;    /* interior cells per axis */
+    double L1, L2, L3;    /* domain lengths (x/y spacing L/nm is uniform) */
+    int pbc1, pbc2, pbc3; /* nonzero: periodic; zero: Neumann walls */
+    int precision;        /* PPF_PREC_FP64 or PPF_PREC_FP32_IR */
+    double tol;           /* FP32_IR: target ||f-Lp||/||f|| (<=0: 1e-10) */
+    int max_sweeps;       /* FP32_IR: refinement sweeps cap (<=0: 8) */
+} ppf_desc;
+
+typedef struct ppf_stats {
+    int sweeps;          /* FP32_IR refinement sweeps taken (0 for FP64) */
+    double rel_residual; /* final ||f-Lp||/||f|| when measured, else -1 */
+} ppf_stats;
+
+/* Plan + buffers + tables for one problem shape.  Replaces
+ * fft_poisson_plan_cuda_create + cuda_Poisson_FFT_initial
+ * (fft_3d_poisson.f90:70-121: BC type, buffers, cuFFT plans, wavenumbers,
+ * PaScaL_TDMA plans).  All device memory the solve needs is allocated here. */
+int ppf_plan_create(const ppf_desc *desc, ppf_plan **plan);
+
+/* One solve.  Replaces cuda_Poisson_FFT_1D(PRHS_d, P_d, dmx1_d, dmx2_d,
+ * dmx3_d, dx3_d) (fft_3d_poisson.f90:353-626).
+ *   prhs  (n1m, n2m, n3m)              right-hand side, NOT modified (the
+ *                                      reference uses it as scratch)
+ *   p     (n1m+2, n2m+2, n3m+2)        solution with the reference's ghost
+ *                                      layers: zero unweighted interior mean
+ *                                      (:628-664), wall extrapolation
+ *                                      (:798-852), periodic wrap (:666-796)
+ *   dmx1, dmx2, dmx3, dx3 (nm+2 each)  grid metrics, element q = the
+ *                                      reference's Fortran element q (0:nsub)
+ *   stream  hipStream_t or NULL
+ *   stats   may be NULL
+ * FP64 plans run asynchronously.  FP32_IR plans with tol > 0 synchronise once
+ * per sweep to test convergence; with tol < 0 they run exactly max_sweeps
+ * sweeps without synchronising. */
+int ppf_solve(ppf_plan *plan, const double *prhs, double *p, const double *dmx1, const double *dmx2,
+              const double *dmx3, const double *dx3, void *stream, ppf_stats *stats);
+
+/* Frees everything the plan owns.  Replaces cuda_Poisson_FFT_clean
+ * (fft_3d_poisson.f90:123-130). */
+int ppf_plan_destroy(ppf_plan *plan);
+
+/* Relative FP64 residual ||f - L p||_2 / ||f||_2 of the 7-point operator the
+ * solve inverts (x/y: uniform spacing with zero-flux walls or wrap, the
+ * operators whose eigenvalues are fft_3d_poisson.f90:288-351; z: the stretched
+ * rows of :1137-1149), on interior cells of p.  Synchronises the stream. */
+int ppf_residual(ppf_plan *plan, const double *f, const double *p, const double *dmx3, const double *dx3,
+                 void *stream, double *rel_residual);
+
+/* Number of device bytes the plan holds (workspace + tables). */
+size_t ppf_plan_workspace_bytes(const ppf_plan *plan);
+
+const char *ppf_error_string(int code);
+
+/* ---- batched Thomas, one rank ------------------------------------------ */
+/* Replaces PaScaL_TDMA_plan_many_create_cuda / _destroy_cuda
+ * (pascal_tdma_cuda.f90:108-220) and, for nprocs = 1,
+ * PaScaL_TDMA_many_solve_cuda (:230-269) -> tdma_many_cuda
+ * (tdmas_cuda.f90:35-102).  Arrays are (nx_sys, ny_sys, nz_row), x fastest;
+ * c and d are overwritten, d holds the solution, with the reference's
+ * operation order (bitwise equal to the reference Thomas on the same inputs).
+ * Unlike the reference, nx_sys and ny_sys need not be multiples of a thread
+ * block (the reference aborts, :168-190). */
+typedef struct ppf_tdma_plan ppf_tdma_plan;
+int ppf_tdma_plan_many_create(int nx_sys, int ny_sys, int nz_row, ppf_tdma_plan **plan);
+int ppf_tdma_many_solve(ppf_tdma_plan *plan, const double *a, const double *b, double *c, double *d, void *stream);
+int ppf_tdma_plan_many_destroy(ppf_tdma_plan *plan);
+
+/* ---- device memory shim for hosts without HIP bindings (Fortran) -------- */
+/* Stand-ins for the CUDA Fortran `device` arrays, allocate/deallocate and
+ * array assignment host<->device the reference relies on
+ * (01_Fortran/examples/cuda_subdomain.f90:96-180). */
+#define PPF_MEMCPY_HOST_TO_DEVICE 1
+#define PPF_MEMCPY_DEVICE_TO_HOST 2
+#define PPF_MEMCPY_DEVICE_TO_DEVICE 3
+int ppf_malloc(void **ptr, size_t bytes);
+int ppf_free(void *ptr);
+int ppf_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int ppf_memset(void *ptr, int value, size_t bytes);
+int ppf_sync(void *stream);
+int ppf_device_count(int *count);
+int ppf_set_device(int device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PASCAL_POISSON_FFT_H */

@@ -1,0 +1,244 @@
+"""Python host side of the solve path, over the C ABI (include/pascal_poisson_fft.h).
+
+Mirrors the reference's module interface (01_Fortran/src/fft_3d_poisson.f90):
+
+    reference                                   here
+    fft_poisson_plan_cuda_create(...) :70       FFTPoisson(n1m, n2m, n3m, L, pbc, ...)
+    cuda_Poisson_FFT_initial()        :110      (done by the constructor)
+    cuda_Poisson_FFT_1D(PRHS_d, P_d, dmx1_d,
+        dmx2_d, dmx3_d, dx3_d)        :353      FFTPoisson.solve(prhs, p, dmx1, dmx2, dmx3, dx3)
+    cuda_Poisson_FFT_clean()          :123      FFTPoisson.clean()
+
+Device memory comes from the library's own shim (DeviceArray), so the process
+carries exactly one HIP runtime.  The library must be built
+(python -m pascal_poisson_fft_amd.build); loading fails loudly otherwise --
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libpascal_poisson_fft.so")
+
+PPF_PREC_FP64 = 0
+PPF_PREC_FP32_IR = 1
+
+H2D, D2H, D2D = 1, 2, 3
+
+
+class PPFError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        self.code = code
+        super().__init__(f"{what}: {error_string(code)} ({code})")
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("n1m", C.c_int), ("n2m", C.c_int), ("n3m", C.c_int),
+        ("L1", C.c_double), ("L2", C.c_double), ("L3", C.c_double),
+        ("pbc1", C.c_int), ("pbc2", C.c_int), ("pbc3", C.c_int),
+        ("precision", C.c_int), ("tol", C.c_double), ("max_sweeps", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [("sweeps", C.c_int), ("rel_residual", C.c_double)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the gfx950 library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m pascal_poisson_fft_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, I, D = C.c_void_p, C.c_size_t, C.c_int, C.c_double
+    L.ppf_plan_create.argtypes = [C.POINTER(Desc), C.POINTER(vp)]
+    L.ppf_solve.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(Stats)]
+    L.ppf_plan_destroy.argtypes = [vp]
+    L.ppf_residual.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(D)]
+    L.ppf_plan_workspace_bytes.argtypes = [vp]
+    L.ppf_plan_workspace_bytes.restype = sz
+    L.ppf_error_string.argtypes = [I]
+    L.ppf_error_string.restype = C.c_char_p
+    L.ppf_tdma_plan_many_create.argtypes = [I, I, I, C.POINTER(vp)]
+    L.ppf_tdma_many_solve.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.ppf_tdma_plan_many_destroy.argtypes = [vp]
+    L.ppf_malloc.argtypes = [C.POINTER(vp), sz]
+    L.ppf_free.argtypes = [vp]
+    L.ppf_memcpy.argtypes = [vp, vp, sz, I]
+    L.ppf_memset.argtypes = [vp, I, sz]
+    L.ppf_sync.argtypes = [vp]
+    L.ppf_device_count.argtypes = [C.POINTER(I)]
+    L.ppf_set_device.argtypes = [I]
+    _lib = L
+    return L
+
+
+def error_string(code: int) -> str:
+    return lib().ppf_error_string(int(code)).decode()
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise PPFError(rc, what)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(lib().ppf_device_count(C.byref(n)), "ppf_device_count")
+    return n.value
+
+
+def sync(stream=None) -> None:
+    _check(lib().ppf_sync(stream), "ppf_sync")
+
+
+class DeviceArray:
+    """A device buffer from ppf_malloc with a numpy shape/dtype for transfers."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(lib().ppf_malloc(C.byref(p), self.nbytes), "ppf_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        _check(lib().ppf_memcpy(self.ptr, a.ctypes.data, self.nbytes, H2D), "ppf_memcpy H2D")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(lib().ppf_memcpy(out.ctypes.data, self.ptr, self.nbytes, D2H), "ppf_memcpy D2H")
+        return out
+
+    def fill_zero(self) -> None:
+        _check(lib().ppf_memset(self.ptr, 0, self.nbytes), "ppf_memset")
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().ppf_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class FFTPoisson:
+    """One plan of the solve path (module fft_poisson of the reference)."""
+
+    def __init__(self, n1m: int, n2m: int, n3m: int, L=(2.0, 2.0, 2.0), pbc=(False, False, False),
+                 precision: int = PPF_PREC_FP64, tol: float = 1e-10, max_sweeps: int = 8):
+        self.desc = Desc(int(n1m), int(n2m), int(n3m), float(L[0]), float(L[1]), float(L[2]),
+                         int(bool(pbc[0])), int(bool(pbc[1])), int(bool(pbc[2])),
+                         int(precision), float(tol), int(max_sweeps))
+        h = C.c_void_p()
+        _check(lib().ppf_plan_create(C.byref(self.desc), C.byref(h)), "ppf_plan_create")
+        self.handle = h.value
+
+    @property
+    def shape(self):
+        return (self.desc.n1m, self.desc.n2m, self.desc.n3m)
+
+    def workspace_bytes(self) -> int:
+        return int(lib().ppf_plan_workspace_bytes(self.handle))
+
+    def solve(self, prhs: DeviceArray, p: DeviceArray, dmx1: DeviceArray, dmx2: DeviceArray, dmx3: DeviceArray,
+              dx3: DeviceArray, stream=None) -> Stats:
+        st = Stats()
+        _check(lib().ppf_solve(self.handle, prhs.ptr, p.ptr, dmx1.ptr, dmx2.ptr, dmx3.ptr, dx3.ptr, stream,
+                               C.byref(st)), "ppf_solve")
+        return st
+
+    def residual(self, f: DeviceArray, p: DeviceArray, dmx3: DeviceArray, dx3: DeviceArray, stream=None) -> float:
+        r = C.c_double()
+        _check(lib().ppf_residual(self.handle, f.ptr, p.ptr, dmx3.ptr, dx3.ptr, stream, C.byref(r)), "ppf_residual")
+        return r.value
+
+    def clean(self) -> None:
+        if self.handle:
+            _check(lib().ppf_plan_destroy(self.handle), "ppf_plan_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.clean()
+        except Exception:
+            pass
+
+
+class TDMAPlanMany:
+    """PaScaL_TDMA_plan_many_create_cuda / PaScaL_TDMA_many_solve_cuda (one rank)."""
+
+    def __init__(self, nx_sys: int, ny_sys: int, nz_row: int):
+        h = C.c_void_p()
+        _check(lib().ppf_tdma_plan_many_create(int(nx_sys), int(ny_sys), int(nz_row), C.byref(h)),
+               "ppf_tdma_plan_many_create")
+        self.handle = h.value
+
+    def solve(self, a: DeviceArray, b: DeviceArray, c: DeviceArray, d: DeviceArray, stream=None) -> None:
+        _check(lib().ppf_tdma_many_solve(self.handle, a.ptr, b.ptr, c.ptr, d.ptr, stream), "ppf_tdma_many_solve")
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib().ppf_tdma_plan_many_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Problem3D:
+    """Device-resident inputs of one problem: RHS, solution buffer and metrics."""
+
+    def __init__(self, rhs: np.ndarray, ax, ay, az):
+        n3m, n2m, n1m = rhs.shape
+        self.rhs = DeviceArray.from_host(rhs)
+        self.p = DeviceArray((n3m + 2, n2m + 2, n1m + 2))
+        self.p.fill_zero()
+        self.dmx1 = DeviceArray.from_host(ax.dmx)
+        self.dmx2 = DeviceArray.from_host(ay.dmx)
+        self.dmx3 = DeviceArray.from_host(az.dmx)
+        self.dx3 = DeviceArray.from_host(az.dx)
+
+    def free(self):
+        for a in (self.rhs, self.p, self.dmx1, self.dmx2, self.dmx3, self.dx3):
+            a.free()
+
+
+def solve_host(rhs: np.ndarray, ax, ay, az, precision: int = PPF_PREC_FP64, **kw) -> np.ndarray:
+    """Convenience: upload, solve once on the GPU, download P (n3m+2, n2m+2, n1m+2)."""
+    n3m, n2m, n1m = rhs.shape
+    plan = FFTPoisson(n1m, n2m, n3m, (ax.L, ay.L, az.L), (ax.pbc, ay.pbc, az.pbc), precision=precision, **kw)
+    prob = Problem3D(rhs, ax, ay, az)
+    plan.solve(prob.rhs, prob.p, prob.dmx1, prob.dmx2, prob.dmx3, prob.dx3)
+    sync()
+    out = prob.p.download()
+    prob.free()
+    plan.clean()
+    return out

@@ -1,0 +1,178 @@
+// ppf_fft.hpp -- LDS-resident complex FFT for gfx950 (Stockham autosort).
+//
+// A workgroup transforms several independent lines at once; each line is M
+// complex values in LDS and is served by TPL threads.  Stages use radix 8, 4,
+// 3 or 2, chosen at compile time from M = 2^a or 3*2^a.  Each stage reads its
+// butterfly inputs into registers, barriers, and writes the outputs back in
+// place (Stockham ordering), so one LDS buffer per line suffices and the
+// result lands in natural order.
+//
+// Twiddles come from a table tw[m] = exp(-2 pi i m / M), m < M, in global
+// memory (read-only, L1/L2 resident).  Forward transforms use sign -1;
+// inverse transforms are obtained by the caller by conjugating on load and on
+// store (IDFT(x) = conj(DFT(conj(x)))).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ppf {
+
+template <typename T>
+struct cx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+// multiply by -i
+template <typename T>
+__device__ __forceinline__ cx<T> cmi(cx<T> a) { return {a.y, -a.x}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cconj(cx<T> a) { return {a.x, -a.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s}; }
+
+// ---- compile-time radix plan -------------------------------------------
+constexpr bool fft_size_ok(int m) {
+    if (m < 1) return false;
+    if (m % 3 == 0) m /= 3;
+    while (m % 2 == 0) m /= 2;
+    return m == 1;
+}
+// radix of the next stage given the remaining length
+constexpr int pick_radix(int rem) {
+    return (rem % 3 == 0) ? 3 : ((rem % 8 == 0 && rem != 16) ? 8 : ((rem % 4 == 0) ? 4 : 2));
+}
+constexpr int max_radix(int m) {
+    int r = 1;
+    while (m > 1) {
+        int p = pick_radix(m);
+        if (p > r) r = p;
+        m /= p;
+    }
+    return r;
+}
+
+// ---- forward DFT butterflies ---------------------------------------------
+template <typename T, int R>
+struct Dft;
+
+template <typename T>
+struct Dft<T, 2> {
+    __device__ __forceinline__ static void run(cx<T> *v) {
+        cx<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <typename T>
+struct Dft<T, 3> {
+    __device__ __forceinline__ static void run(cx<T> *v) {
+        const T h = T(0.86602540378443864676372317075294);  // sqrt(3)/2
+        cx<T> a = v[0], b = v[1], c = v[2];
+        cx<T> s = cadd(b, c);
+        cx<T> t = {a.x - T(0.5) * s.x, a.y - T(0.5) * s.y};
+        cx<T> z = {h * (b.x - c.x), h * (b.y - c.y)};
+        v[0] = cadd(a, s);
+        v[1] = cadd(t, cmi(z));
+        v[2] = csub(t, cmi(z));
+    }
+};
+
+template <typename T>
+struct Dft<T, 4> {
+    __device__ __forceinline__ static void run(cx<T> *v) {
+        cx<T> s0 = cadd(v[0], v[2]), d0 = csub(v[0], v[2]);
+        cx<T> s1 = cadd(v[1], v[3]), d1 = cmi(csub(v[1], v[3]));
+        v[0] = cadd(s0, s1);
+        v[2] = csub(s0, s1);
+        v[1] = cadd(d0, d1);
+        v[3] = csub(d0, d1);
+    }
+};
+
+template <typename T>
+struct Dft<T, 8> {
+    __device__ __forceinline__ static void run(cx<T> *v) {
+        const T r = T(0.70710678118654752440084436210485);
+        cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+        cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<T, 4>::run(e);
+        Dft<T, 4>::run(o);
+        // o_k *= exp(-2 pi i k / 8)
+        cx<T> o1 = {r * (o[1].x + o[1].y), r * (o[1].y - o[1].x)};
+        cx<T> o2 = cmi(o[2]);
+        cx<T> o3 = {r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y)};
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+};
+
+// ---- one Stockham stage ---------------------------------------------------
+// line: LDS base of this thread's line (M complex), t: thread within the line.
+template <typename T, int M, int TPL, int R, int LS>
+__device__ __forceinline__ void fft_stage(cx<T> *line, int t, const cx<T> *__restrict__ tw) {
+    constexpr int S = M / R;                  // butterflies per line
+    constexpr int NB = (S + TPL - 1) / TPL;   // butterflies per thread
+    constexpr int STEP = M / (LS * R);        // twiddle table stride
+    cx<T> v[NB][R];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int b = t + q * TPL;
+        if ((S % TPL == 0) || b < S) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[q][r] = line[b + r * S];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int b = t + q * TPL;
+        if ((S % TPL == 0) || b < S) {
+            const int j = b % LS;
+            const int kk = b / LS;
+            if (LS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], tw[j * r * STEP]);
+            }
+            Dft<T, R>::run(v[q]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) line[kk * LS * R + j + r * LS] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int M, int TPL, int LS>
+struct FftStages {
+    __device__ __forceinline__ static void run(cx<T> *line, int t, const cx<T> *__restrict__ tw) {
+        constexpr int R = pick_radix(M / LS);
+        fft_stage<T, M, TPL, R, LS>(line, t, tw);
+        FftStages<T, M, TPL, LS * R>::run(line, t, tw);
+    }
+};
+template <typename T, int M, int TPL>
+struct FftStages<T, M, TPL, M> {
+    __device__ __forceinline__ static void run(cx<T> *, int, const cx<T> *__restrict__) {}
+};
+
+// Forward FFT (sign -1) of one LDS line, in place, natural order in and out.
+// Every thread of the workgroup must call it (it contains barriers).
+template <typename T, int M, int TPL>
+__device__ __forceinline__ void fft_forward(cx<T> *line, int t, const cx<T> *__restrict__ tw) {
+    static_assert(fft_size_ok(M), "FFT length must be 2^a or 3*2^a");
+    FftStages<T, M, TPL, 1>::run(line, t, tw);
+}
+
+}  // namespace ppf

@@ -1,0 +1,401 @@
+// ppf_solver.hip -- plan, tables, launch sequence and C ABI of the solve path.
+//
+// Replaces module fft_poisson of 01_Fortran/src/fft_3d_poisson.f90 (plan
+// create :70-108, initial :110-121, solve :353-626, clean :123-130) for one
+// GPU.  The reference's ~57 field passes (transposes, DCT pre/post kernels,
+// cuFFT calls, coefficient arrays, dcopy/dscal, mean and BC sweeps) become the
+// five passes of ppf_kernels.hpp.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/pascal_poisson_fft.h"
+#include "ppf_kernels.hpp"
+
+using namespace ppf;
+
+namespace {
+
+const double kPi = 3.14159265358979323846264338327950288;
+
+// Transform kind along one axis.
+enum AxisKind { AX_DCT = 0, AX_R2C = 1, AX_C2C = 2 };
+
+template <typename T>
+struct AxisTables {
+    cx<T> *tw = nullptr;  // FFT twiddles, M entries
+    cx<T> *wk = nullptr;  // exp(-2 pi i k / N), M+1 entries
+    cx<T> *qk = nullptr;  // exp(-i pi k / 2N), M+1 entries
+};
+
+bool size_supported(int n) {
+    if (n < 8 || n > 1024 || (n & 1)) return false;
+    int m = n;
+    if (m % 3 == 0) m /= 3;
+    while (m % 2 == 0) m /= 2;
+    return m == 1;
+}
+
+}  // namespace
+
+struct ppf_plan {
+    ppf_desc d;
+    int Lx, Ly, Lz;        // workspace extents (scalars)
+    int xkind, ykind;      // AxisKind
+    int ymode;             // y pass MODE (0 DCT, 1 C2C, 2 R2C)
+    int zjlo, znjr;        // rows solved by the z pass
+    int mean_kx, mean_jp;
+    double mean_scale;
+    // device memory
+    double *W = nullptr;   // spectral workspace (FP64 path)
+    double *cs = nullptr;  // Thomas scratch
+    double *lamx = nullptr, *lamy = nullptr;
+    unsigned char *zx = nullptr, *zy = nullptr;
+    double *mean = nullptr;     // device scalar
+    double *partial = nullptr;  // residual partial sums
+    double *red = nullptr;      // residual totals (2)
+    int npartial = 0;
+    AxisTables<double> tx64, ty64;
+    size_t bytes = 0;
+};
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+#define PPF_HIP(call)                                  \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return PPF_ERR_HIP;      \
+    } while (0)
+
+template <typename T>
+static int dev_alloc(ppf_plan *p, T **ptr, size_t count) {
+    void *q = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc(&q, count * sizeof(T)) != hipSuccess) return PPF_ERR_ALLOC;
+    *ptr = static_cast<T *>(q);
+    p->bytes += count * sizeof(T);
+    return PPF_OK;
+}
+
+template <typename T>
+static int upload_tables(ppf_plan *p, AxisTables<T> &tb, int N, int kind) {
+    const int M = (kind == AX_C2C) ? N : N / 2;
+    std::vector<cx<T>> tw(M), wk(M + 1), qk(M + 1);
+    for (int m = 0; m < M; ++m) {
+        long double a = -2.0L * (long double)kPi * (long double)m / (long double)M;
+        tw[m] = {(T)cosl(a), (T)sinl(a)};
+    }
+    for (int k = 0; k <= M; ++k) {
+        long double a = -2.0L * (long double)kPi * (long double)k / (long double)N;
+        wk[k] = {(T)cosl(a), (T)sinl(a)};
+        long double b = -(long double)kPi * (long double)k / (2.0L * (long double)N);
+        qk[k] = {(T)cosl(b), (T)sinl(b)};
+    }
+    int rc;
+    if ((rc = dev_alloc(p, &tb.tw, M)) || (rc = dev_alloc(p, &tb.wk, M + 1)) || (rc = dev_alloc(p, &tb.qk, M + 1)))
+        return rc;
+    PPF_HIP(hipMemcpy(tb.tw, tw.data(), sizeof(cx<T>) * M, hipMemcpyHostToDevice));
+    PPF_HIP(hipMemcpy(tb.wk, wk.data(), sizeof(cx<T>) * (M + 1), hipMemcpyHostToDevice));
+    PPF_HIP(hipMemcpy(tb.qk, qk.data(), sizeof(cx<T>) * (M + 1), hipMemcpyHostToDevice));
+    return PPF_OK;
+}
+
+// modified wavenumbers, fft_3d_poisson.f90:288-351
+static double eig_neumann(int m, int n, double h) {
+    double s = std::sin(0.5 * (double)m * kPi / (double)n);
+    return 4.0 * s * s / (h * h);
+}
+static double eig_periodic(int m, int n, double h) {
+    return 2.0 * (1.0 - std::cos(2.0 * (double)m * kPi / (double)n)) / (h * h);
+}
+
+// ---------------------------------------------------------------------------
+// kernel dispatch over the supported line lengths
+// ---------------------------------------------------------------------------
+#define PPF_FOR_SIZES(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+
+template <typename T, int N>
+static void launch_xfwd(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, hipStream_t s) {
+    const int nlines = p->d.n2m * p->d.n3m;
+    if (p->xkind == AX_R2C) {
+        using X = XGeom<T, N, true>;
+        hipLaunchKernelGGL((k_xfwd<T, N, true>), dim3((nlines + X::LPB - 1) / X::LPB), dim3(X::NT), 0, s, in, W,
+                           nlines, p->d.n2m, p->Ly, tb.tw, tb.wk, tb.qk);
+    } else {
+        using X = XGeom<T, N, false>;
+        hipLaunchKernelGGL((k_xfwd<T, N, false>), dim3((nlines + X::LPB - 1) / X::LPB), dim3(X::NT), 0, s, in, W,
+                           nlines, p->d.n2m, p->Ly, tb.tw, tb.wk, tb.qk);
+    }
+}
+
+template <typename T, int N>
+static void launch_xinv(const ppf_plan *p, const T *W, double *P, const AxisTables<T> &tb, const double *dmx1,
+                        hipStream_t s) {
+    const int nlines = p->Ly * p->Lz;
+    if (p->xkind == AX_R2C) {
+        using X = XGeom<T, N, true>;
+        hipLaunchKernelGGL((k_xinv<T, N, true>), dim3((nlines + X::LPB - 1) / X::LPB), dim3(X::NT), 0, s, W, P,
+                           nlines, tb.tw, tb.wk, tb.qk, dmx1, p->mean);
+    } else {
+        using X = XGeom<T, N, false>;
+        hipLaunchKernelGGL((k_xinv<T, N, false>), dim3((nlines + X::LPB - 1) / X::LPB), dim3(X::NT), 0, s, W, P,
+                           nlines, tb.tw, tb.wk, tb.qk, dmx1, p->mean);
+    }
+}
+
+template <typename T, int N, int MODE>
+static void launch_y_mode(const ppf_plan *p, T *W, const AxisTables<T> &tb, bool inverse, const double *dmx2,
+                          hipStream_t s) {
+    using Y = YGeom<T, N, MODE>;
+    const int nb = (p->Lx + Y::TX - 1) / Y::TX;
+    if (!inverse)
+        hipLaunchKernelGGL((k_yfwd<T, N, MODE>), dim3(nb, p->d.n3m), dim3(Y::NT), 0, s, W, p->Lx, p->Ly, tb.tw, tb.wk,
+                           tb.qk);
+    else
+        hipLaunchKernelGGL((k_yinv<T, N, MODE>), dim3(nb, p->Lz), dim3(Y::NT), 0, s, W, p->Lx, p->Ly, tb.tw, tb.wk,
+                           tb.qk, dmx2);
+}
+
+template <typename T, int N>
+static void launch_y(const ppf_plan *p, T *W, const AxisTables<T> &tb, bool inverse, const double *dmx2,
+                     hipStream_t s) {
+    if (p->ymode == 0) launch_y_mode<T, N, 0>(p, W, tb, inverse, dmx2, s);
+    else if (p->ymode == 1) launch_y_mode<T, N, 1>(p, W, tb, inverse, dmx2, s);
+    else launch_y_mode<T, N, 2>(p, W, tb, inverse, dmx2, s);
+}
+
+template <typename T>
+static int dispatch_xfwd(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, hipStream_t s) {
+    switch (p->d.n1m) {
+#define X_CASE(n) \
+    case n: launch_xfwd<T, n>(p, in, W, tb, s); return PPF_OK;
+        PPF_FOR_SIZES(X_CASE)
+#undef X_CASE
+    }
+    return PPF_ERR_SIZE;
+}
+template <typename T>
+static int dispatch_xinv(const ppf_plan *p, const T *W, double *P, const AxisTables<T> &tb, const double *dmx1,
+                         hipStream_t s) {
+    switch (p->d.n1m) {
+#define X_CASE(n) \
+    case n: launch_xinv<T, n>(p, W, P, tb, dmx1, s); return PPF_OK;
+        PPF_FOR_SIZES(X_CASE)
+#undef X_CASE
+    }
+    return PPF_ERR_SIZE;
+}
+template <typename T>
+static int dispatch_y(const ppf_plan *p, T *W, const AxisTables<T> &tb, bool inverse, const double *dmx2,
+                      hipStream_t s) {
+    switch (p->d.n2m) {
+#define Y_CASE(n) \
+    case n: launch_y<T, n>(p, W, tb, inverse, dmx2, s); return PPF_OK;
+        PPF_FOR_SIZES(Y_CASE)
+#undef Y_CASE
+    }
+    return PPF_ERR_SIZE;
+}
+
+static int launch_z(ppf_plan *p, double *W, const double *dx3, const double *dmx3, hipStream_t s) {
+    ZArgs z;
+    z.Lx = p->Lx;
+    z.Ly = p->Ly;
+    z.n3m = p->d.n3m;
+    z.jlo = p->zjlo;
+    z.njr = p->znjr;
+    z.lamx = p->lamx;
+    z.lamy = p->lamy;
+    z.zx = p->zx;
+    z.zy = p->zy;
+    z.mean_kx = p->mean_kx;
+    z.mean_jp = p->mean_jp;
+    z.mean_scale = p->mean_scale;
+    const int n3m = p->d.n3m;
+    const long ncol = (long)p->Lx * p->znjr;
+    const int nt = 256;
+    const size_t shm = 2 * sizeof(double) * (size_t)n3m;
+    hipLaunchKernelGGL((k_zthomas<double>), dim3((unsigned)((ncol + nt - 1) / nt)), dim3(nt), shm, s, W, p->cs, z, dx3,
+                       dmx3, p->mean);
+    return PPF_OK;
+}
+
+static void free_plan(ppf_plan *p) {
+    if (!p) return;
+    void *ptrs[] = {p->W, p->cs, p->lamx, p->lamy, p->zx, p->zy, p->mean, p->partial, p->red,
+                    p->tx64.tw, p->tx64.wk, p->tx64.qk, p->ty64.tw, p->ty64.wk, p->ty64.qk};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    delete p;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *ppf_error_string(int code) {
+    switch (code) {
+        case PPF_OK: return "success";
+        case PPF_ERR_INVALID_ARG: return "invalid argument";
+        case PPF_ERR_SIZE: return "unsupported extent (x/y need even 8..1024 of the form 2^a or 3*2^a; z >= 2 even)";
+        case PPF_ERR_HIP: return "HIP runtime error";
+        case PPF_ERR_ALLOC: return "device allocation failed";
+        case PPF_ERR_UNSUPPORTED: return "unsupported configuration";
+        case PPF_ERR_NOT_CONVERGED: return "refinement did not reach the tolerance";
+    }
+    return "unknown error";
+}
+
+int ppf_plan_create(const ppf_desc *desc, ppf_plan **out) {
+    if (!desc || !out) return PPF_ERR_INVALID_ARG;
+    *out = nullptr;
+    const ppf_desc &d = *desc;
+    if (d.n1m <= 0 || d.n2m <= 0 || d.n3m <= 0 || !(d.L1 > 0) || !(d.L2 > 0) || !(d.L3 > 0)) return PPF_ERR_INVALID_ARG;
+    if (d.precision != PPF_PREC_FP64 && d.precision != PPF_PREC_FP32_IR) return PPF_ERR_INVALID_ARG;
+    // every n must be even (01_Fortran/examples/global.f90:114-120)
+    // z rows live in LDS in the z pass (2 doubles per row): n3m <= 4096
+    if (!size_supported(d.n1m) || !size_supported(d.n2m) || d.n3m < 2 || (d.n3m & 1) || d.n3m > 4096)
+        return PPF_ERR_SIZE;
+    if (d.pbc3) return PPF_ERR_UNSUPPORTED;  // cyclic z Thomas not provided
+    if (d.precision != PPF_PREC_FP64) return PPF_ERR_UNSUPPORTED;
+
+    ppf_plan *p = new (std::nothrow) ppf_plan();
+    if (!p) return PPF_ERR_ALLOC;
+    p->d = d;
+    const bool px = d.pbc1 != 0, py = d.pbc2 != 0;
+    p->xkind = px ? AX_R2C : AX_DCT;
+    if (!py) { p->ykind = AX_DCT; p->ymode = 0; }
+    else if (px) { p->ykind = AX_C2C; p->ymode = 1; }
+    else { p->ykind = AX_R2C; p->ymode = 2; }
+    p->Lx = px ? d.n1m + 2 : d.n1m;
+    p->Ly = d.n2m + 2;
+    p->Lz = d.n3m + 2;
+    if (p->ymode == 2) { p->zjlo = 0; p->znjr = d.n2m + 2; }
+    else { p->zjlo = 1; p->znjr = d.n2m; }
+    p->mean_kx = 0;
+    p->mean_jp = (p->ymode == 2) ? 0 : 1;
+    const double fx = px ? 1.0 : 2.0, fy = (p->ymode == 0) ? 2.0 : 1.0;
+    p->mean_scale = 1.0 / (fx * fy * (double)d.n1m * (double)d.n2m * (double)d.n3m);
+
+    int rc = PPF_OK;
+    const size_t wsz = (size_t)p->Lx * p->Ly * p->Lz;
+    const size_t csz = (size_t)p->Lx * p->znjr * d.n3m;
+    if ((rc = dev_alloc(p, &p->W, wsz)) || (rc = dev_alloc(p, &p->cs, csz)) || (rc = dev_alloc(p, &p->lamx, p->Lx)) ||
+        (rc = dev_alloc(p, &p->lamy, p->Ly)) || (rc = dev_alloc(p, &p->zx, p->Lx)) || (rc = dev_alloc(p, &p->zy, p->Ly)) ||
+        (rc = dev_alloc(p, &p->mean, 1)) || (rc = dev_alloc(p, &p->red, 2))) {
+        free_plan(p);
+        return rc;
+    }
+    p->npartial = 1024;
+    if ((rc = dev_alloc(p, &p->partial, 2 * (size_t)p->npartial))) { free_plan(p); return rc; }
+
+    // eigenvalue and zero-mode tables per scalar column / row
+    const double h1 = d.L1 / d.n1m, h2 = d.L2 / d.n2m;
+    std::vector<double> lx(p->Lx), ly(p->Ly, 0.0);
+    std::vector<unsigned char> zx(p->Lx, 0), zy(p->Ly, 0);
+    for (int kx = 0; kx < p->Lx; ++kx) {
+        const int m = px ? kx / 2 : kx;
+        lx[kx] = px ? eig_periodic(m, d.n1m, h1) : eig_neumann(m, d.n1m, h1);
+        zx[kx] = (m == 0);
+    }
+    for (int jp = 0; jp < p->Ly; ++jp) {
+        int m;
+        if (p->ymode == 0) m = jp - 1;
+        else if (p->ymode == 1) { const int j = jp - 1; m = (j <= d.n2m / 2) ? j : d.n2m - j; }
+        else m = jp / 2;
+        if (m < 0 || m > d.n2m) { ly[jp] = 0.0; zy[jp] = 0; continue; }  // unused ghost rows
+        if (p->ymode == 0 && jp > d.n2m) { ly[jp] = 0.0; zy[jp] = 0; continue; }
+        if (p->ymode == 1 && (jp < 1 || jp > d.n2m)) { ly[jp] = 0.0; zy[jp] = 0; continue; }
+        ly[jp] = (p->ymode == 0) ? eig_neumann(m, d.n2m, h2) : eig_periodic(m, d.n2m, h2);
+        zy[jp] = (m == 0);
+    }
+    if (hipMemcpy(p->lamx, lx.data(), sizeof(double) * p->Lx, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->lamy, ly.data(), sizeof(double) * p->Ly, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->zx, zx.data(), p->Lx, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->zy, zy.data(), p->Ly, hipMemcpyHostToDevice) != hipSuccess) {
+        free_plan(p);
+        return PPF_ERR_HIP;
+    }
+    if ((rc = upload_tables(p, p->tx64, d.n1m, p->xkind)) || (rc = upload_tables(p, p->ty64, d.n2m, p->ykind))) {
+        free_plan(p);
+        return rc;
+    }
+    *out = p;
+    return PPF_OK;
+}
+
+int ppf_plan_destroy(ppf_plan *plan) {
+    if (!plan) return PPF_ERR_INVALID_ARG;
+    free_plan(plan);
+    return PPF_OK;
+}
+
+size_t ppf_plan_workspace_bytes(const ppf_plan *plan) { return plan ? plan->bytes : 0; }
+
+int ppf_solve(ppf_plan *p, const double *prhs, double *P, const double *dmx1, const double *dmx2, const double *dmx3,
+              const double *dx3, void *stream, ppf_stats *stats) {
+    if (!p || !prhs || !P || !dmx1 || !dmx2 || !dmx3 || !dx3) return PPF_ERR_INVALID_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc;
+    if ((rc = dispatch_xfwd<double>(p, prhs, p->W, p->tx64, s))) return rc;
+    if ((rc = dispatch_y<double>(p, p->W, p->ty64, false, dmx2, s))) return rc;
+    if ((rc = launch_z(p, p->W, dx3, dmx3, s))) return rc;
+    if ((rc = dispatch_y<double>(p, p->W, p->ty64, true, dmx2, s))) return rc;
+    if ((rc = dispatch_xinv<double>(p, p->W, P, p->tx64, dmx1, s))) return rc;
+    PPF_HIP(hipGetLastError());
+    if (stats) { stats->sweeps = 0; stats->rel_residual = -1.0; }
+    return PPF_OK;
+}
+
+int ppf_residual(ppf_plan *p, const double *f, const double *P, const double *dmx3, const double *dx3, void *stream,
+                 double *rel) {
+    if (!p || !f || !P || !dmx3 || !dx3 || !rel) return PPF_ERR_INVALID_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ResArgs a;
+    a.n1m = p->d.n1m; a.n2m = p->d.n2m; a.n3m = p->d.n3m;
+    a.pbc1 = p->d.pbc1 != 0; a.pbc2 = p->d.pbc2 != 0;
+    const double h1 = p->d.L1 / p->d.n1m, h2 = p->d.L2 / p->d.n2m;
+    a.ih1 = 1.0 / (h1 * h1);
+    a.ih2 = 1.0 / (h2 * h2);
+    hipLaunchKernelGGL(k_residual, dim3(p->npartial), dim3(256), 0, s, f, P, a, dx3, dmx3, (double *)nullptr, p->partial);
+    hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(256), 0, s, p->partial, p->npartial, p->red);
+    double h[2];
+    PPF_HIP(hipMemcpyAsync(h, p->red, sizeof(h), hipMemcpyDeviceToHost, s));
+    PPF_HIP(hipStreamSynchronize(s));
+    *rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+    return PPF_OK;
+}
+
+// ---- device memory shim ------------------------------------------------------
+int ppf_malloc(void **ptr, size_t bytes) {
+    if (!ptr) return PPF_ERR_INVALID_ARG;
+    return hipMalloc(ptr, bytes ? bytes : 1) == hipSuccess ? PPF_OK : PPF_ERR_ALLOC;
+}
+int ppf_free(void *ptr) { return hipFree(ptr) == hipSuccess ? PPF_OK : PPF_ERR_HIP; }
+int ppf_memcpy(void *dst, const void *src, size_t bytes, int kind) {
+    hipMemcpyKind k;
+    switch (kind) {
+        case PPF_MEMCPY_HOST_TO_DEVICE: k = hipMemcpyHostToDevice; break;
+        case PPF_MEMCPY_DEVICE_TO_HOST: k = hipMemcpyDeviceToHost; break;
+        case PPF_MEMCPY_DEVICE_TO_DEVICE: k = hipMemcpyDeviceToDevice; break;
+        default: return PPF_ERR_INVALID_ARG;
+    }
+    return hipMemcpy(dst, src, bytes, k) == hipSuccess ? PPF_OK : PPF_ERR_HIP;
+}
+int ppf_memset(void *ptr, int value, size_t bytes) { return hipMemset(ptr, value, bytes) == hipSuccess ? PPF_OK : PPF_ERR_HIP; }
+int ppf_sync(void *stream) {
+    return hipStreamSynchronize(static_cast<hipStream_t>(stream)) == hipSuccess ? PPF_OK : PPF_ERR_HIP;
+}
+int ppf_device_count(int *count) {
+    if (!count) return PPF_ERR_INVALID_ARG;
+    return hipGetDeviceCount(count) == hipSuccess ? PPF_OK : PPF_ERR_HIP;
+}
+int ppf_set_device(int device) { return hipSetDevice(device) == hipSuccess ? PPF_OK : PPF_ERR_HIP; }
+
+}  // extern "C"

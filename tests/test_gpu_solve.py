@@ -1,0 +1,137 @@
+"""GPU parity: the HIP solve path (through the C ABI) against the FP64 oracle.
+
+Tolerance (FP64 path): relative L2 of the full ghosted solution P against the
+oracle <= 1e-12, per the plain-method drop-in target; the FP64 residual of the
+GPU solution <= 1e-11.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pascal_poisson_fft_amd import api
+from pascal_poisson_fft_amd import workloads as W
+from pascal_poisson_fft_amd.grid import make_axis
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-12
+
+
+def _rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def _problem(n1, n2, n3, pbc, uniform_z, gamma, seed, L=(2.0, 3.0, 1.5), compatible=True):
+    ax = make_axis(n1, L[0], pbc[0])
+    ay = make_axis(n2, L[1], pbc[1])
+    az = make_axis(n3, L[2], False, uniform=uniform_z, gamma=gamma)
+    f = np.random.default_rng(seed).standard_normal((n3, n2, n1))
+    if compatible:
+        f = W.remove_weighted_mean(f, az)
+    return ax, ay, az, f
+
+
+BCS = [(False, False), (True, True), (True, False), (False, True)]
+
+
+@pytest.mark.parametrize("pbc", BCS, ids=["NN", "PP", "PN", "NP"])
+@pytest.mark.parametrize("n1,n2,n3", [(8, 8, 8), (16, 12, 10), (32, 48, 20), (64, 64, 64), (96, 24, 6), (12, 192, 34)])
+def test_solve_matches_oracle(pbc, n1, n2, n3):
+    ax, ay, az, f = _problem(n1, n2, n3, pbc, False, 1.5, n1 * 7 + n2 * 3 + n3)
+    ref = O.solve(f, (ax.L, ay.L, az.L), (pbc[0], pbc[1], False), ax.dmx, ay.dmx, az.dmx, az.dx)
+    got = api.solve_host(f, ax, ay, az)
+    assert got.shape == ref.shape
+    assert _rel_l2(got, ref) <= TOL_FP64, _rel_l2(got, ref)
+
+
+@pytest.mark.parametrize("pbc", BCS, ids=["NN", "PP", "PN", "NP"])
+def test_incompatible_rhs_pin_semantics(pbc):
+    """RHS without the compatibility projection: the (0,0)-mode pin decides the
+    answer; it must match the oracle exactly as the reference's pin does."""
+    ax, ay, az, f = _problem(24, 16, 18, pbc, True, 0.0, 11, compatible=False)
+    f = f + 0.3
+    ref = O.solve(f, (ax.L, ay.L, az.L), (pbc[0], pbc[1], False), ax.dmx, ay.dmx, az.dmx, az.dx)
+    got = api.solve_host(f, ax, ay, az)
+    assert _rel_l2(got, ref) <= TOL_FP64
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_solve_matches_oracle_large(n):
+    prob = W.smoothed_normal((n, n, n // 2), (False, False, False), 1.5, seed=5)
+    ax, ay, az = prob.ax
+    ref = O.solve(prob.rhs, prob.L, prob.pbc, ax.dmx, ay.dmx, az.dmx, az.dx)
+    got = api.solve_host(prob.rhs, ax, ay, az)
+    assert _rel_l2(got, ref) <= TOL_FP64
+
+
+def test_c1_manufactured_64():
+    prob = W.manufactured_c1(64)
+    ax, ay, az = prob.ax
+    got = api.solve_host(prob.rhs, ax, ay, az)
+    ref = O.solve(prob.rhs, prob.L, prob.pbc, ax.dmx, ay.dmx, az.dmx, az.dx)
+    assert _rel_l2(got, ref) <= TOL_FP64
+    assert np.abs(got[1:-1, 1:-1, 1:-1] - prob.exact).max() < 1e-3
+    assert O.rel_residual(prob.rhs, got, prob.L, prob.pbc, az.dmx, az.dx) < 1e-11
+
+
+def test_gpu_residual_matches_oracle():
+    ax, ay, az, f = _problem(32, 24, 16, (True, False), False, 1.2, 3)
+    P = np.random.default_rng(4).standard_normal((18, 26, 34))
+    plan = api.FFTPoisson(32, 24, 16, (ax.L, ay.L, az.L), (True, False, False))
+    fd, pd = api.DeviceArray.from_host(f), api.DeviceArray.from_host(P)
+    d3, x3 = api.DeviceArray.from_host(az.dmx), api.DeviceArray.from_host(az.dx)
+    r = plan.residual(fd, pd, d3, x3)
+    ref = O.rel_residual(f, P, (ax.L, ay.L, az.L), (True, False), az.dmx, az.dx)
+    assert abs(r - ref) <= 1e-12 * ref
+    plan.clean()
+
+
+def test_repeated_solves_are_deterministic_and_leave_rhs_intact():
+    ax, ay, az, f = _problem(64, 32, 40, (False, False), False, 1.5, 8)
+    plan = api.FFTPoisson(64, 32, 40, (ax.L, ay.L, az.L), (False, False, False))
+    prob = api.Problem3D(f, ax, ay, az)
+    plan.solve(prob.rhs, prob.p, prob.dmx1, prob.dmx2, prob.dmx3, prob.dx3)
+    api.sync()
+    a = prob.p.download()
+    prob.p.upload(np.full(a.shape, 7.0))  # stale ghosts/values must not leak in
+    plan.solve(prob.rhs, prob.p, prob.dmx1, prob.dmx2, prob.dmx3, prob.dx3)
+    api.sync()
+    b = prob.p.download()
+    assert np.array_equal(a, b)
+    assert np.array_equal(prob.rhs.download(), f)
+    prob.free()
+    plan.clean()
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(32, 8, 64), (7, 3, 5), (100, 1, 2), (512, 4, 512)])
+def test_tdma_many_bitwise_vs_reference_order(nx, ny, nz):
+    rng = np.random.default_rng(nx + ny + nz)
+    a = rng.uniform(0.1, 1.0, (nz, ny, nx))
+    c = rng.uniform(0.1, 1.0, (nz, ny, nx))
+    b = -(a + c) - rng.uniform(0.0, 2.0, (nz, ny, nx))
+    a[0] = 0.0
+    c[-1] = 0.0
+    d = rng.standard_normal((nz, ny, nx))
+    ref = O.tdma_many(a.reshape(nz, -1), b.reshape(nz, -1), c.reshape(nz, -1), d.reshape(nz, -1)).reshape(d.shape)
+    if O.ref_tdma_available():
+        ref2 = O.ref_tdma_many(a.reshape(nz, -1), b.reshape(nz, -1), c.reshape(nz, -1), d.reshape(nz, -1))
+        assert np.array_equal(ref, ref2.reshape(d.shape))
+    plan = api.TDMAPlanMany(nx, ny, nz)
+    A, B, Cd, D = (api.DeviceArray.from_host(x) for x in (a, b, c, d))
+    plan.solve(A, B, Cd, D)
+    api.sync()
+    got = D.download()
+    assert np.array_equal(got, ref)
+    plan.destroy()
+
+
+def test_plan_rejects_bad_sizes():
+    with pytest.raises(api.PPFError) as e:
+        api.FFTPoisson(10, 8, 8)  # 10 = 2*5 is not 2^a or 3*2^a
+    assert e.value.code == -2
+    with pytest.raises(api.PPFError) as e:
+        api.FFTPoisson(8, 8, 7)  # odd z
+    assert e.value.code == -2
+    with pytest.raises(api.PPFError) as e:
+        api.FFTPoisson(8, 8, 8, pbc=(False, False, True))
+    assert e.value.code == -5
