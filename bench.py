@@ -1,0 +1,174 @@
+#!/usr/bin/env python
+"""Benchmark of the solve path on one MI355X.
+
+Workload: 512^3 cells, FP64, Neumann x and y (DCT-II/III), tanh-stretched z
+(stretch 1.5) with Neumann walls, L = 2; RHS = seeded N(0,1) field smoothed by
+one 3x3x3 box mean with the dx3-weighted mean removed (synthetic, seed 2).
+A step is one full solve (RHS in HBM -> ghosted solution in HBM).
+
+Prints one JSON line: throughput in Mpts/s (cells / solve time), the per-pass
+HBM roofline of the dominant pass (algorithmic bytes / pass time from HIP
+events on the solve stream), and the FP64 CPU oracle timed on a slab of the
+same workload on the host cores as a reported baseline.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--n 512] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "512^3 solve ms + Mpts/s & %HBM roofline, FP32+refine vs FP64 at 1e-10 resid"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PT_STAGE = 16.0  # one FP64 read + one FP64 write per cell per pass
+BYTES_PER_PT_SOLVE = 80.0  # five passes
+
+
+def cpu_baseline(n: int, nz_sample: int, reps: int = 5):
+    """FP64 CPU oracle (the project's restatement of the reference method) on an
+    n x n x nz_sample slab of the same workload; median of `reps` after 1 warm-up."""
+    from oracle import oracle as O
+    from pascal_poisson_fft_amd import workloads as W
+
+    prob = W.smoothed_normal((n, n, nz_sample), (False, False, False), 1.5, seed=2)
+    ax, ay, az = prob.ax
+    args = (prob.rhs, prob.L, prob.pbc, ax.dmx, ay.dmx, az.dmx, az.dx)
+    t0 = time.perf_counter()
+    O.solve(*args)
+    first = time.perf_counter() - t0
+    reps = max(1, min(reps, int(20.0 / max(first, 1e-3))))
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.solve(*args)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    npts = n * n * nz_sample
+    return {
+        "value": round(npts / t / 1e6, 3),
+        "unit": "Mpts/s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": f"FP64 oracle (C restatement of the reference solve, OpenMP) on a {n}x{n}x{nz_sample} slab "
+                  f"of the same workload, median of {reps} after 1 warm-up ({t:.3f} s/solve)",
+    }
+
+
+def load_traffic(kernel_key: str):
+    """HBM bytes per launch of the dominant pass from committed rocprofv3 PMC
+    summaries (profiles/traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+        ent = data.get(kernel_key)
+        return None if ent is None else float(ent["bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-nz", type=int, default=0, help="z extent of the CPU baseline slab (default n/4)")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the z-slab multi-GPU decomposition is a design note only)", file=sys.stderr)
+        return 2
+
+    from pascal_poisson_fft_amd import api
+    from pascal_poisson_fft_amd import workloads as W
+
+    n = a.n
+    t0 = time.perf_counter()
+    prob = W.config2(n)
+    gen_s = time.perf_counter() - t0
+    ax, ay, az = prob.ax
+    plan = api.FFTPoisson(n, n, n, prob.L, prob.pbc)
+    dev = api.Problem3D(prob.rhs, ax, ay, az)
+    solve_args = (dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3)
+    print(f"# problem ready ({gen_s:.1f} s host generation); warmup {a.warmup}", file=sys.stderr, flush=True)
+
+    for _ in range(a.warmup):
+        plan.solve(*solve_args)
+    api.sync()
+
+    api.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        plan.solve(*solve_args)
+    api.sync()
+    wall = time.perf_counter() - t0
+    ms = wall / a.steps * 1e3
+    print(f"# timed {a.steps} solves: {ms:.3f} ms/solve", file=sys.stderr, flush=True)
+
+    # per-pass times (HIP events on the solve stream), same workload
+    stages = np.array([plan.solve_timed(*solve_args) for _ in range(max(3, a.steps))])
+    stage_ms = stages.mean(axis=0)
+    rel_res = plan.residual(dev.rhs, dev.p, dev.dmx3, dev.dx3)
+    npts = n ** 3
+    dom = int(np.argmax(stage_ms))
+    dom_name = api.FFTPoisson.STAGES[dom]
+    achieved = BYTES_PER_PT_STAGE * npts / (stage_ms[dom] * 1e-3) / 1e9
+    traffic = load_traffic(dom_name)
+    out = {
+        "metric": METRIC,
+        "value": round(npts / (ms * 1e-3) / 1e6, 2),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{n}^3 FP64 Neumann x,y (DCT-II/III), tanh z stretch 1.5 with Neumann walls, L=2, "
+                        "smoothed N(0,1) RHS seed 2 with dx3-weighted mean removed",
+            "shape": [n, n, n],
+            "precision": "fp64",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": BYTES_PER_PT_STAGE * npts,
+        },
+        "solve_roofline_frac": round(BYTES_PER_PT_SOLVE * npts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "stage_ms": {k: round(float(v), 4) for k, v in zip(api.FFTPoisson.STAGES, stage_ms)},
+        "rel_residual": rel_res,
+    }
+    dev.free()
+    plan.clean()
+    if not a.no_cpu:
+        nz = a.cpu_nz or max(2, n // 4)
+        print(f"# cpu baseline on {n}x{n}x{nz}", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(n, nz)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

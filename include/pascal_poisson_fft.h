@@ -81,6 +81,14 @@ int ppf_plan_create(const ppf_desc *desc, ppf_plan **plan);
 int ppf_solve(ppf_plan *plan, const double *prhs, double *p, const double *dmx1, const double *dmx2,
               const double *dmx3, const double *dx3, void *stream, ppf_stats *stats);
 
+/* ppf_solve with HIP events recorded on `stream` around each pass; on return
+ * (the stream is synchronised) stage_ms[0..4] hold the durations of the
+ * x-forward, y-forward, z-Thomas, y-inverse and x-inverse passes.  The
+ * counterpart of the reference's NVTX ranges Poisson-F_x, -F_y, -TDMA-z,
+ * -B_y, -B_x (fft_3d_poisson.f90:377-624), for measurement. */
+int ppf_solve_timed(ppf_plan *plan, const double *prhs, double *p, const double *dmx1, const double *dmx2,
+                    const double *dmx3, const double *dx3, void *stream, float stage_ms[5]);
+
 /* Frees everything the plan owns.  Replaces cuda_Poisson_FFT_clean
  * (fft_3d_poisson.f90:123-130). */
 int ppf_plan_destroy(ppf_plan *plan);

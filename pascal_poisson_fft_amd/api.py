@@ -63,6 +63,7 @@ def lib() -> C.CDLL:
     vp, sz, I, D = C.c_void_p, C.c_size_t, C.c_int, C.c_double
     L.ppf_plan_create.argtypes = [C.POINTER(Desc), C.POINTER(vp)]
     L.ppf_solve.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(Stats)]
+    L.ppf_solve_timed.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(C.c_float)]
     L.ppf_plan_destroy.argtypes = [vp]
     L.ppf_residual.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(D)]
     L.ppf_plan_workspace_bytes.argtypes = [vp]
@@ -171,6 +172,16 @@ class FFTPoisson:
         _check(lib().ppf_solve(self.handle, prhs.ptr, p.ptr, dmx1.ptr, dmx2.ptr, dmx3.ptr, dx3.ptr, stream,
                                C.byref(st)), "ppf_solve")
         return st
+
+    STAGES = ("x-forward", "y-forward", "z-thomas", "y-inverse", "x-inverse")
+
+    def solve_timed(self, prhs: DeviceArray, p: DeviceArray, dmx1: DeviceArray, dmx2: DeviceArray,
+                    dmx3: DeviceArray, dx3: DeviceArray, stream=None) -> list:
+        """One solve with HIP events around each pass; returns the 5 pass times in ms."""
+        ms = (C.c_float * 5)()
+        _check(lib().ppf_solve_timed(self.handle, prhs.ptr, p.ptr, dmx1.ptr, dmx2.ptr, dmx3.ptr, dx3.ptr, stream,
+                                     ms), "ppf_solve_timed")
+        return [float(v) for v in ms]
 
     def residual(self, f: DeviceArray, p: DeviceArray, dmx3: DeviceArray, dx3: DeviceArray, stream=None) -> float:
         r = C.c_double()

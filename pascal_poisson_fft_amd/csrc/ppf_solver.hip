@@ -353,6 +353,35 @@ int ppf_solve(ppf_plan *p, const double *prhs, double *P, const double *dmx1, co
     return PPF_OK;
 }
 
+int ppf_solve_timed(ppf_plan *p, const double *prhs, double *P, const double *dmx1, const double *dmx2,
+                    const double *dmx3, const double *dx3, void *stream, float stage_ms[5]) {
+    if (!p || !prhs || !P || !dmx1 || !dmx2 || !dmx3 || !dx3 || !stage_ms) return PPF_ERR_INVALID_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t ev[6];
+    for (int i = 0; i < 6; ++i) PPF_HIP(hipEventCreate(&ev[i]));
+    int rc = PPF_OK;
+    (void)hipEventRecord(ev[0], s);
+    if (!rc) rc = dispatch_xfwd<double>(p, prhs, p->W, p->tx64, s);
+    (void)hipEventRecord(ev[1], s);
+    if (!rc) rc = dispatch_y<double>(p, p->W, p->ty64, false, dmx2, s);
+    (void)hipEventRecord(ev[2], s);
+    if (!rc) rc = launch_z(p, p->W, dx3, dmx3, s);
+    (void)hipEventRecord(ev[3], s);
+    if (!rc) rc = dispatch_y<double>(p, p->W, p->ty64, true, dmx2, s);
+    (void)hipEventRecord(ev[4], s);
+    if (!rc) rc = dispatch_xinv<double>(p, p->W, P, p->tx64, dmx1, s);
+    (void)hipEventRecord(ev[5], s);
+    if (hipEventSynchronize(ev[5]) != hipSuccess && !rc) rc = PPF_ERR_HIP;
+    for (int i = 0; i < 5; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess && !rc) rc = PPF_ERR_HIP;
+        stage_ms[i] = ms;
+    }
+    for (int i = 0; i < 6; ++i) (void)hipEventDestroy(ev[i]);
+    if (!rc && hipGetLastError() != hipSuccess) rc = PPF_ERR_HIP;
+    return rc;
+}
+
 int ppf_residual(ppf_plan *p, const double *f, const double *P, const double *dmx3, const double *dx3, void *stream,
                  double *rel) {
     if (!p || !f || !P || !dmx3 || !dx3 || !rel) return PPF_ERR_INVALID_ARG;
