@@ -21,6 +21,26 @@ namespace ppf {
 
 constexpr int kMaxNT = 512;
 
+// LDS layout of the line-transform kernels: NL lines of LSD scalars each
+// (LSD odd, so consecutive lines start on different banks and the
+// column-to-line scatter of the y passes is conflict-free), followed by the
+// axis tables tw[M] | wk[M+1] | qk[M+1], copied from global memory on entry
+// together with the data so no FFT stage waits on a global round trip.
+template <typename T, int M, int S, int NL>
+struct LdsLines {
+    static constexpr int LSD = 2 * S + 1;
+    static constexpr int TABN = M + 2 * (M + 1);
+    static constexpr int SIZE = NL * LSD + 2 * TABN;  // scalars of type T
+    __device__ __forceinline__ static cx<T> *line(T *lds, int l) { return reinterpret_cast<cx<T> *>(lds + l * LSD); }
+    __device__ __forceinline__ static T *real(T *lds, int l) { return lds + l * LSD; }
+    __device__ __forceinline__ static cx<T> *tables(T *lds) { return reinterpret_cast<cx<T> *>(lds + NL * LSD); }
+    __device__ __forceinline__ static void load_tables(T *lds, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
+                                                       const cx<T> *__restrict__ qk, int tid, int nt) {
+        cx<T> *tab = tables(lds);
+        for (int i = tid; i < TABN; i += nt) tab[i] = i < M ? tw[i] : (i <= 2 * M ? wk[i - M] : qk[i - 2 * M - 1]);
+    }
+};
+
 // ---------------------------------------------------------------------------
 // pass 1: x-forward.  One LDS line per prhs row; LPB rows per workgroup.
 // PER: periodic x (R2C, Lx = N+2) else DCT-II (Lx = N).
@@ -31,41 +51,81 @@ struct XGeom {
     static constexpr int TPL = G::TPL;
     static constexpr int LPB = (256 / TPL) > 0 ? (256 / TPL) : 1;
     static constexpr int NT = LPB * TPL;
+    using L = LdsLines<T, G::M, G::STRIDE, LPB>;
 };
 
+// q = a / b for 0 <= a < 2^24 via a float reciprocal and one correction
+__device__ __forceinline__ int fdiv_small(int a, int b, float rb) {
+    int q = (int)((float)a * rb);
+    q -= (q * b > a);
+    q += ((q + 1) * b <= a);
+    return q;
+}
+
+// The line-transform kernels are persistent and software-pipelined: a
+// workgroup walks tiles tile = blockIdx.x, blockIdx.x + gridDim.x, ...; the
+// next tile's global loads are issued into registers right after the current
+// tile has been scattered into LDS, so they are in flight while the current
+// tile is transformed and stored (barriers wait on LDS only).
 template <typename T, int N, bool PER>
 __global__ void __launch_bounds__(256)
 k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, int Ly,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk) {
     using X = XGeom<T, N, PER>;
     using G = typename X::G;
-    constexpr int M = G::M, TPL = X::TPL, LPB = X::LPB, NT = X::NT, S = G::STRIDE;
+    using L = typename X::L;
+    constexpr int M = G::M, TPL = X::TPL, LPB = X::LPB, NT = X::NT;
     constexpr int LX = PER ? N + 2 : N;
-    __shared__ cx<T> lds[LPB * S];
+    constexpr int EPT = (LPB * N + NT - 1) / NT;  // loads per thread per tile
+    __shared__ T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
-    const int line0 = blockIdx.x * LPB;
-    for (int e = tid; e < LPB * N; e += NT) {
-        const int li = e / N, n = e - li * N;
-        const int line = line0 + li;
-        T v = T(0);
-        if (line < nlines) v = (T)in[(size_t)line * N + n];
-        T *re = reinterpret_cast<T *>(lds + li * S);
-        re[PER ? n : makhoul_pos<N>(n)] = v;
-    }
-    __syncthreads();
-    cx<T> *my = lds + l * S;
-    fft_forward<T, M, TPL>(my, t, tw);
-    if (PER)
-        r2c_post<T, N, TPL>(my, t, wk);
-    else
-        dct_post<T, N, TPL>(my, t, wk, qk);
-    for (int e = tid; e < LPB * LX; e += NT) {
-        const int li = e / LX, n = e - li * LX;
-        const int line = line0 + li;
-        if (line < nlines) {
-            const int j = line % n2m, k = line / n2m;
-            const T *re = reinterpret_cast<const T *>(lds + li * S);
-            W[((size_t)(k + 1) * Ly + (j + 1)) * LX + n] = re[n];
+    const int ntiles = (nlines + LPB - 1) / LPB;
+    const float rn2m = 1.0f / (float)n2m;
+    double pre[EPT];
+    // unconditional loads from clamped addresses (lines past the end are
+    // transformed but never stored); nothing consumes them before the next
+    // tile, so they stay in flight
+    auto fetch = [&](int tile) {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            int e = tid + q * NT;
+            e = e < LPB * N ? e : LPB * N - 1;
+            const int li = e / N, n = e - li * N;
+            int line = tile * LPB + li;
+            line = line < nlines ? line : nlines - 1;
+            pre[q] = in[(size_t)line * N + n];
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile);
+    L::load_tables(lds, tw, wk, qk, tid, NT);
+    const cx<T> *tab = L::tables(lds);
+    cx<T> *my = L::line(lds, l);
+    for (; tile < ntiles; tile += gridDim.x) {
+        __syncthreads();  // the previous tile's LDS lines have been stored
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = tid + q * NT;
+            if (e < LPB * N) {
+                const int li = e / N, n = e - li * N;
+                L::real(lds, li)[PER ? n : makhoul_pos<N>(n)] = (T)pre[q];
+            }
+        }
+        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+        __syncthreads();
+        fft_forward<T, M, TPL>(my, t, tab);
+        if (PER)
+            r2c_post<T, N, TPL>(my, t, tab + M);
+        else
+            dct_post<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
+        for (int e = tid; e < LPB * LX; e += NT) {
+            const int li = e / LX, n = e - li * LX;
+            const int line = tile * LPB + li;
+            if (line < nlines) {
+                // W row of prhs line (j,k): (k+1)*Ly + j + 1 = line + n2m + 3 + 2k
+                const int k = fdiv_small(line, n2m, rn2m);
+                W[((size_t)line + (size_t)(Ly + 1) + 2 * (size_t)k) * LX + n] = L::real(lds, li)[n];
+            }
         }
     }
 }
@@ -81,42 +141,64 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
        const double *__restrict__ dmx1, const double *__restrict__ mean) {
     using X = XGeom<T, N, PER>;
     using G = typename X::G;
-    constexpr int M = G::M, TPL = X::TPL, LPB = X::LPB, NT = X::NT, S = G::STRIDE;
+    using L = typename X::L;
+    constexpr int M = G::M, TPL = X::TPL, LPB = X::LPB, NT = X::NT;
     constexpr int LX = PER ? N + 2 : N;
     constexpr int LP = N + 2;
-    __shared__ cx<T> lds[LPB * S];
+    constexpr int EPT = (LPB * LX + NT - 1) / NT;
+    __shared__ T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
-    const int line0 = blockIdx.x * LPB;
-    for (int e = tid; e < LPB * LX; e += NT) {
-        const int li = e / LX, n = e - li * LX;
-        const int line = line0 + li;
-        T v = T(0);
-        if (line < nlines) v = W[(size_t)line * LX + n];
-        reinterpret_cast<T *>(lds + li * S)[n] = v;
-    }
-    __syncthreads();
-    cx<T> *my = lds + l * S;
-    if (PER)
-        c2r_pre<T, N, TPL>(my, t, wk);
-    else
-        idct_pre<T, N, TPL>(my, t, wk, qk);
-    fft_forward<T, M, TPL>(my, t, tw);
-    const double scale = PER ? 1.0 / (double)N : 1.0 / (2.0 * (double)N);
+    const int ntiles = (nlines + LPB - 1) / LPB;
+    T pre[EPT];
+    auto fetch = [&](int tile) {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            int e = tid + q * NT;
+            e = e < LPB * LX ? e : LPB * LX - 1;
+            size_t idx = (size_t)tile * LPB * LX + e;
+            const size_t last = (size_t)nlines * LX - 1;
+            pre[q] = W[idx < last ? idx : last];
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile);
+    L::load_tables(lds, tw, wk, qk, tid, NT);
     const double mu = *mean;
     double bca0 = 0, bcb0 = 0, bca1 = 0, bcb1 = 0;
     if (!PER) {
-        const double d1 = dmx1[1], d2 = dmx1[2], s0 = d1 + d2;
+        const double d1 = dmx1[1], d2 = dmx1[2], e1 = dmx1[N + 1], e2 = dmx1[N];
+        const double s0 = d1 + d2;
         bca0 = s0 * s0 / (s0 * s0 - d1 * d1);
         bcb0 = d1 * d1 / (s0 * s0 - d1 * d1);
-        const double e1 = dmx1[N + 1], e2 = dmx1[N], s1 = e1 + e2;
+        const double s1 = e1 + e2;
         bca1 = s1 * s1 / (s1 * s1 - e1 * e1);
         bcb1 = e1 * e1 / (s1 * s1 - e1 * e1);
     }
+    const double scale = PER ? 1.0 / (double)N : 1.0 / (2.0 * (double)N);
+    const cx<T> *tab = L::tables(lds);
+    cx<T> *my = L::line(lds, l);
+    for (; tile < ntiles; tile += gridDim.x) {
+    __syncthreads();  // the previous tile's LDS lines have been stored
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = tid + q * NT;
+        if (e < LPB * LX) {
+            const int li = e / LX, n = e - li * LX;
+            L::real(lds, li)[n] = pre[q];
+        }
+    }
+    if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+    __syncthreads();
+    if (PER)
+        c2r_pre<T, N, TPL>(my, t, tab + M);
+    else
+        idct_pre<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
+    fft_forward<T, M, TPL>(my, t, tab);
     for (int e = tid; e < LPB * LP; e += NT) {
         const int li = e / LP, np = e - li * LP;
-        const int line = line0 + li;
+        const int line = tile * LPB + li;
         if (line >= nlines) continue;
-        const T *re = reinterpret_cast<const T *>(lds + li * S);
+        const T *re = L::real(lds, li);
         // physical sample n (0-based) of this line, mean removed
         auto sample = [&](int n) -> double {
             const int tpos = PER ? n : makhoul_pos<N>(n);
@@ -132,6 +214,7 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
             out = sample(np - 1);
         P[(size_t)line * LP + np] = out;
     }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -145,47 +228,75 @@ struct YGeom {
     static constexpr int M = CPLX ? N : N / 2;
     static constexpr int TPL = (M / max_radix(M)) > 0 ? (M / max_radix(M)) : 1;
     static constexpr int S = CPLX ? M + 1 : (MODE == 2 ? M + 1 : M);
-    static constexpr int LINE_BYTES = S * (int)sizeof(cx<T>);
-    // scalar columns per tile: 128-byte rows, LDS kept within 64 KiB,
-    // threads within kMaxNT
+    // scalar columns per tile: 128-byte rows, LDS (lines + tables) within
+    // 96 KiB, threads within kMaxNT
+    static constexpr int lds_bytes(int tx) {
+        return ((CPLX ? tx / 2 : tx) * (2 * S + 1) + 2 * (M + 2 * (M + 1))) * (int)sizeof(T);
+    }
     static constexpr int TX0 = 128 / (int)sizeof(T);
     static constexpr int tx_fit(int tx) {
-        return (tx > 2 && ((CPLX ? tx / 2 : tx) * LINE_BYTES > 65536 || (CPLX ? tx / 2 : tx) * TPL > kMaxNT))
-                   ? tx_fit(tx / 2)
-                   : tx;
+        return (tx > 2 && (lds_bytes(tx) > 98304 || (CPLX ? tx / 2 : tx) * TPL > kMaxNT)) ? tx_fit(tx / 2) : tx;
     }
     static constexpr int TX = tx_fit(TX0);
     static constexpr int NL = CPLX ? TX / 2 : TX;  // LDS lines per tile
     static constexpr int NT = NL * TPL;
+    using L = LdsLines<T, M, S, NL>;
 };
 
 template <typename T, int N, int MODE>
 __global__ void __launch_bounds__(kMaxNT)
-k_yfwd(T *__restrict__ W, int Lx, int Ly, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
+k_yfwd(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
        const cx<T> *__restrict__ qk) {
     using Y = YGeom<T, N, MODE>;
-    constexpr int M = Y::M, TPL = Y::TPL, TX = Y::TX, NT = Y::NT, S = Y::S;
-    __shared__ cx<T> lds[Y::NL * S];
+    using L = typename Y::L;
+    constexpr int M = Y::M, TPL = Y::TPL, TX = Y::TX, NT = Y::NT;
+    constexpr int EPT = (TX * N + NT - 1) / NT;
+    __shared__ T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
-    const int c0 = blockIdx.x * TX;
-    const size_t plane = (size_t)(blockIdx.y + 1) * Ly;
-    for (int e = tid; e < TX * N; e += NT) {
-        const int r = e / TX, c = e - r * TX;
-        const int col = c0 + c;
-        T v = T(0);
-        if (col < Lx) v = W[(plane + 1 + r) * Lx + col];
-        if (MODE == 0)
-            reinterpret_cast<T *>(lds + c * S)[makhoul_pos<N>(r)] = v;
-        else if (MODE == 1)
-            reinterpret_cast<T *>(lds + (c >> 1) * S)[2 * r + (c & 1)] = v;
-        else
-            reinterpret_cast<T *>(lds + c * S)[r] = v;
+    const int nct = (Lx + TX - 1) / TX;
+    const int ntiles = nct * nplanes;
+    T pre[EPT];
+    auto fetch = [&](int tile) {  // unconditional, clamped: see k_xfwd
+        const int ct = tile % nct, pl = tile / nct;
+        const size_t plane = (size_t)(pl + 1) * Ly;
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            int e = tid + q * NT;
+            e = e < TX * N ? e : TX * N - 1;
+            const int r = e / TX, c = e - r * TX;
+            int col = ct * TX + c;
+            col = col < Lx ? col : Lx - 1;
+            pre[q] = W[(plane + 1 + r) * Lx + col];
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile);
+    L::load_tables(lds, tw, wk, qk, tid, NT);
+    const cx<T> *tab = L::tables(lds);
+    cx<T> *my = L::line(lds, l);
+    for (; tile < ntiles; tile += gridDim.x) {
+    const int c0 = (tile % nct) * TX;
+    const size_t plane = (size_t)(tile / nct + 1) * Ly;
+    __syncthreads();  // the previous tile's LDS lines have been stored
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = tid + q * NT;
+        if (e < TX * N) {
+            const int r = e / TX, c = e - r * TX;
+            const T v = pre[q];
+            if (MODE == 0)
+                L::real(lds, c)[makhoul_pos<N>(r)] = v;
+            else if (MODE == 1)
+                L::real(lds, c >> 1)[2 * r + (c & 1)] = v;
+            else
+                L::real(lds, c)[r] = v;
+        }
     }
+    if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
     __syncthreads();
-    cx<T> *my = lds + l * S;
-    fft_forward<T, M, TPL>(my, t, tw);
-    if (MODE == 0) dct_post<T, N, TPL>(my, t, wk, qk);
-    if (MODE == 2) r2c_post<T, N, TPL>(my, t, wk);
+    fft_forward<T, M, TPL>(my, t, tab);
+    if (MODE == 0) dct_post<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
+    if (MODE == 2) r2c_post<T, N, TPL>(my, t, tab + M);
     constexpr int NR = MODE == 2 ? N + 2 : N;  // spectral rows written
     constexpr int R0 = MODE == 2 ? 0 : 1;
     for (int e = tid; e < TX * NR; e += NT) {
@@ -194,62 +305,91 @@ k_yfwd(T *__restrict__ W, int Lx, int Ly, const cx<T> *__restrict__ tw, const cx
         if (col >= Lx) continue;
         T v;
         if (MODE == 1)
-            v = reinterpret_cast<const T *>(lds + (c >> 1) * S)[2 * r + (c & 1)];
+            v = L::real(lds, c >> 1)[2 * r + (c & 1)];
         else
-            v = reinterpret_cast<const T *>(lds + c * S)[r];
+            v = L::real(lds, c)[r];
         W[(plane + R0 + r) * Lx + col] = v;
+    }
     }
 }
 
 template <typename T, int N, int MODE>
 __global__ void __launch_bounds__(kMaxNT)
-k_yinv(T *__restrict__ W, int Lx, int Ly, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
+k_yinv(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
        const cx<T> *__restrict__ qk, const double *__restrict__ dmx2) {
     using Y = YGeom<T, N, MODE>;
-    constexpr int M = Y::M, TPL = Y::TPL, TX = Y::TX, NT = Y::NT, S = Y::S;
-    __shared__ cx<T> lds[Y::NL * S];
-    const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
-    const int c0 = blockIdx.x * TX;
-    const size_t plane = (size_t)blockIdx.y * Ly;  // every plane incl. z ghosts
+    using L = typename Y::L;
+    constexpr int M = Y::M, TPL = Y::TPL, TX = Y::TX, NT = Y::NT;
     constexpr int NR = MODE == 2 ? N + 2 : N;
     constexpr int R0 = MODE == 2 ? 0 : 1;
-    for (int e = tid; e < TX * NR; e += NT) {
-        const int r = e / TX, c = e - r * TX;
-        const int col = c0 + c;
-        T v = T(0);
-        if (col < Lx) v = W[(plane + R0 + r) * Lx + col];
-        if (MODE == 1)  // conjugate on load (inverse via the forward FFT)
-            reinterpret_cast<T *>(lds + (c >> 1) * S)[2 * r + (c & 1)] = (c & 1) ? -v : v;
-        else
-            reinterpret_cast<T *>(lds + c * S)[r] = v;
-    }
-    __syncthreads();
-    cx<T> *my = lds + l * S;
-    if (MODE == 0) idct_pre<T, N, TPL>(my, t, wk, qk);
-    if (MODE == 2) c2r_pre<T, N, TPL>(my, t, wk);
-    fft_forward<T, M, TPL>(my, t, tw);
+    constexpr int EPT = (TX * NR + NT - 1) / NT;
+    __shared__ T lds[L::SIZE];
+    const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
+    const int nct = (Lx + TX - 1) / TX;
+    const int ntiles = nct * nplanes;  // every plane incl. the z ghost planes
+    T pre[EPT];
+    auto fetch = [&](int tile) {  // unconditional, clamped: see k_xfwd
+        const int ct = tile % nct, pl = tile / nct;
+        const size_t plane = (size_t)pl * Ly;
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            int e = tid + q * NT;
+            e = e < TX * NR ? e : TX * NR - 1;
+            const int r = e / TX, c = e - r * TX;
+            int col = ct * TX + c;
+            col = col < Lx ? col : Lx - 1;
+            pre[q] = W[(plane + R0 + r) * Lx + col];
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile);
+    L::load_tables(lds, tw, wk, qk, tid, NT);
     const T scale = MODE == 0 ? T(1.0 / (2.0 * N)) : T(1.0 / (double)N);
     // y ghosts: Neumann for MODE 0, periodic wrap otherwise
     T bca0 = 0, bcb0 = 0, bca1 = 0, bcb1 = 0;
     if (MODE == 0) {
-        const double d1 = dmx2[1], d2 = dmx2[2], s0 = d1 + d2;
+        const double d1 = dmx2[1], d2 = dmx2[2], e1 = dmx2[N + 1], e2 = dmx2[N];
+        const double s0 = d1 + d2;
         bca0 = (T)(s0 * s0 / (s0 * s0 - d1 * d1));
         bcb0 = (T)(d1 * d1 / (s0 * s0 - d1 * d1));
-        const double e1 = dmx2[N + 1], e2 = dmx2[N], s1 = e1 + e2;
+        const double s1 = e1 + e2;
         bca1 = (T)(s1 * s1 / (s1 * s1 - e1 * e1));
         bcb1 = (T)(e1 * e1 / (s1 * s1 - e1 * e1));
     }
+    const cx<T> *tab = L::tables(lds);
+    cx<T> *my = L::line(lds, l);
+    for (; tile < ntiles; tile += gridDim.x) {
+    const int c0 = (tile % nct) * TX;
+    const size_t plane = (size_t)(tile / nct) * Ly;
+    __syncthreads();  // the previous tile's LDS lines have been stored
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+        const int e = tid + q * NT;
+        if (e < TX * NR) {
+            const int r = e / TX, c = e - r * TX;
+            const T v = pre[q];
+            if (MODE == 1)  // conjugate on load (inverse via the forward FFT)
+                L::real(lds, c >> 1)[2 * r + (c & 1)] = (c & 1) ? -v : v;
+            else
+                L::real(lds, c)[r] = v;
+        }
+    }
+    if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+    __syncthreads();
+    if (MODE == 0) idct_pre<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
+    if (MODE == 2) c2r_pre<T, N, TPL>(my, t, tab + M);
+    fft_forward<T, M, TPL>(my, t, tab);
     for (int e = tid; e < TX * (N + 2); e += NT) {
         const int rp = e / TX, c = e - rp * TX;
         const int col = c0 + c;
         if (col >= Lx) continue;
         auto sample = [&](int n) -> T {  // physical row n (0-based)
             if (MODE == 1) {
-                const T v = reinterpret_cast<const T *>(lds + (c >> 1) * S)[2 * n + (c & 1)];
+                const T v = L::real(lds, c >> 1)[2 * n + (c & 1)];
                 return ((c & 1) ? -v : v) * scale;
             }
             const int tpos = MODE == 0 ? makhoul_pos<N>(n) : n;
-            const T v = reinterpret_cast<const T *>(lds + c * S)[tpos];
+            const T v = L::real(lds, c)[tpos];
             return ((tpos & 1) ? -v : v) * scale;
         };
         T out;
@@ -260,6 +400,7 @@ k_yinv(T *__restrict__ W, int Lx, int Ly, const cx<T> *__restrict__ tw, const cx
         else
             out = sample(rp - 1);
         W[(plane + rp) * Lx + col] = out;
+    }
     }
 }
 
@@ -353,6 +494,474 @@ k_zthomas(T *__restrict__ W, T *__restrict__ cs, ZArgs z, const double *__restri
     w[0] = (T)(bca0 * x1 - bcb0 * x2);
     w[(size_t)(n + 1) * pstride] = (T)(bca1 * xn - bcb1 * xm1);
     if (kx == z.mean_kx && jp == z.mean_jp) *mean_out = sum * z.mean_scale;
+}
+
+// ---------------------------------------------------------------------------
+// pass 3, register-resident partitioned Thomas.
+//
+// A workgroup owns S adjacent spectral columns of one row jp and splits each
+// z line into P partitions of K rows; thread (s, p) keeps its K values in
+// registers, so the line is read once and written once.  Per partition (rows
+// l = 0..K-1, outside neighbours x_{-1}, x_K):
+//   reverse sweep (upper diagonal eliminated towards row 0, x_{K-1} kept):
+//       row 0 becomes  ah x_{-1} + x_0 + ch x_{K-1} = dh
+//   forward sweep (lower diagonal eliminated, x_0 kept), rows l >= 1:
+//       A_l x_0 + x_l + C_l x_{l+1} = D_l,  row K-1 couples to x_K
+// These two rows of every partition form a 2P-row tridiagonal system in the
+// partition end values (the reduced system of PaScaL_TDMA's modified Thomas,
+// pascal_tdma_cuda.f90:332-473 and :230-269 for many ranks), solved per column
+// in LDS.  With x_0 and x_{K-1} known, a Thomas sweep over rows 1..K-2 that
+// reuses the forward multipliers C_l recovers the interior.  Both reduction
+// sweeps carry O(1) state, so a thread holds 2K values (its right-hand sides
+// and C).  Rows past n3m are identity rows.  Coefficients, pin, ghost planes
+// and mean as in k_zthomas.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double frcp(double x) {
+    // reciprocal: hardware estimate + two Newton steps (about 1 ulp)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// z row coefficients for one solve: zam[k], zap[k] (k < n3m) as in
+// fft_3d_poisson.f90:1143-1146, then zap[n3m..n3m+3] = BC_a, BC_b at the
+// bottom wall and at the top wall (:829-837).
+__global__ void __launch_bounds__(256)
+k_zcoef(const double *__restrict__ dx3, const double *__restrict__ dmx3, int n3m, double *__restrict__ zam,
+        double *__restrict__ zap) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n3m) {
+        const int kk = k + 1;
+        zam[k] = kk == 1 ? 0.0 : 1.0 / dx3[kk] / dmx3[kk];
+        zap[k] = kk == n3m ? 0.0 : 1.0 / dx3[kk] / dmx3[kk + 1];
+    }
+    if (k == 0) {
+        const double d1 = dmx3[1], d2 = dmx3[2], s0 = d1 + d2;
+        zap[n3m] = s0 * s0 / (s0 * s0 - d1 * d1);
+        zap[n3m + 1] = d1 * d1 / (s0 * s0 - d1 * d1);
+        const double e1 = dmx3[n3m + 1], e2 = dmx3[n3m], s1 = e1 + e2;
+        zap[n3m + 2] = s1 * s1 / (s1 * s1 - e1 * e1);
+        zap[n3m + 3] = e1 * e1 / (s1 * s1 - e1 * e1);
+    }
+}
+
+// WPE: minimum waves per SIMD the register allocation must allow (occupancy
+// is what lets one workgroup's loads overlap another's arithmetic here).
+template <typename T, int K, int S, int NTMAX, int WPE>
+__global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double *__restrict__ zap,
+        const double *__restrict__ dmx3, double *__restrict__ mean_out, int P) {
+    extern __shared__ double zsh[];
+    const int n3m = z.n3m;
+    const int NP = P * K;           // padded rows
+    double *am = zsh;               // [NP], 0 past n3m
+    double *ap = am + NP;           // [NP], 0 past n3m
+    double *rl = ap + NP;           // reduced lower   [2P][S]
+    double *ru = rl + 2 * P * S;    // reduced upper
+    double *rd = ru + 2 * P * S;    // reduced rhs / solution
+    double *ps = rd + 2 * P * S;    // reduced diagonal [2P][S], later partial sums [P][S]
+    const int tid = threadIdx.x, s = tid % S, p = tid / S;
+    const int tiles = (z.Lx + S - 1) / S;
+    const int jp = z.jlo + blockIdx.x / tiles;
+    const int kx = (blockIdx.x % tiles) * S + s;
+    const bool active = kx < z.Lx;
+    const size_t pstride = (size_t)z.Ly * z.Lx;
+    T *w = W + (size_t)jp * z.Lx + (active ? kx : 0);
+    const double lx = active ? z.lamx[kx] : 0.0;
+    const double ly = z.lamy[jp];
+    const bool pin = active && z.zx[kx] && z.zy[jp];
+    const int g0 = p * K;
+    // loads are clamped in bounds and selected, so the sweep code is branch-free
+    const int glast = n3m - 1;
+    // all loads first (clamped addresses, no consumer), masks afterwards
+    double d[K];
+#pragma unroll
+    for (int l = 0; l < K; ++l) {
+        const int g = g0 + l;
+        d[l] = (double)w[(size_t)((g < glast ? g : glast) + 1) * pstride];
+    }
+#pragma unroll
+    for (int l = 0; l < K; ++l) d[l] = (active && g0 + l < n3m) ? d[l] : 0.0;
+    if (pin && g0 == 0) d[0] = 0.0;
+    // row coefficients (precomputed once per solve by k_zcoef) go to LDS while
+    // the line loads above are still in flight
+    for (int k = tid; k < NP; k += blockDim.x) {
+        am[k] = k < n3m ? zam[k] : 0.0;
+        ap[k] = k < n3m ? zap[k] : 0.0;
+    }
+    double bc[4];  // z wall extrapolation, also precomputed by k_zcoef
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bc[q] = zap[n3m + q];
+    __syncthreads();
+    // identity rows: padding past n3m, and the pinned first row of the (0,0) mode
+    auto coef = [&](int l, double &a, double &b, double &c) {
+        const int g = g0 + l;
+        const bool id = (g >= n3m) || (pin && g == 0);
+        const double am_ = am[g], ap_ = ap[g];
+        a = am_;
+        c = id ? 0.0 : ap_;
+        b = id ? 1.0 : (-am_ - ap_ - lx) - ly;
+    };
+    // Reverse sweep (first-row equation) and forward sweep (last-row
+    // equation) interleaved: two independent dependency chains per step.
+    // The forward pivots r_l are kept for the final pass.
+    double rr[K];
+    double ah, ch, dh, Af, Df, Cf;
+    {
+        double a, b, c;
+        coef(K - 2, a, b, c);
+        double r = frcp(b);
+        ah = r * a; ch = r * c; dh = r * d[K - 2];
+        coef(1, a, b, c);
+        r = frcp(b);
+        rr[1] = r;
+        Af = r * a; Cf = r * c; Df = r * d[1];
+#pragma unroll
+        for (int i = 1; i <= K - 2; ++i) {
+            const int lb = K - 2 - i;  // reverse row
+            const int lf = 1 + i;      // forward row
+            double a1, b1, c1, a2, b2, c2;
+            coef(lb, a1, b1, c1);
+            coef(lf, a2, b2, c2);
+            const double r1 = frcp(b1 - c1 * ah);
+            const double r2 = frcp(b2 - a2 * Cf);
+            ch = -r1 * c1 * ch;
+            dh = r1 * (d[lb] - c1 * dh);
+            ah = r1 * a1;
+            rr[lf] = r2;
+            Af = -r2 * a2 * Af;
+            Cf = r2 * c2;
+            Df = r2 * (d[lf] - a2 * Df);
+            // keep coefficient reads next to their use (register pressure)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // reduced system: rows 2p (first-row equation) and 2p+1 (last-row
+    // equation) of column s; unit diagonal
+    rl[(2 * p) * S + s] = ah;
+    ru[(2 * p) * S + s] = ch;
+    rd[(2 * p) * S + s] = dh;
+    rl[(2 * p + 1) * S + s] = Af;
+    ru[(2 * p + 1) * S + s] = Cf;
+    rd[(2 * p + 1) * S + s] = Df;
+    double *rb = ps;  // diagonal during the reduction
+    rb[(2 * p) * S + s] = 1.0;
+    rb[(2 * p + 1) * S + s] = 1.0;
+    __syncthreads();
+    // parallel cyclic reduction: every thread owns rows 2p and 2p+1 of its
+    // column; ceil(log2 2P) steps, rows outside [0, 2P) are identity rows
+    {
+        const int R = 2 * P;
+        for (int h = 1; h < R; h <<= 1) {
+            double nl[2], nb[2], nu[2], nd[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int i = 2 * p + q;
+                const double li = rl[i * S + s], bi = rb[i * S + s], ui = ru[i * S + s], di = rd[i * S + s];
+                double al = 0.0, ga = 0.0;
+                double lm = 0.0, um = 0.0, dm = 0.0, lp = 0.0, up = 0.0, dpp = 0.0;
+                if (i - h >= 0) {
+                    const int j = i - h;
+                    al = -li * frcp(rb[j * S + s]);
+                    lm = rl[j * S + s]; um = ru[j * S + s]; dm = rd[j * S + s];
+                }
+                if (i + h < R) {
+                    const int j = i + h;
+                    ga = -ui * frcp(rb[j * S + s]);
+                    lp = rl[j * S + s]; up = ru[j * S + s]; dpp = rd[j * S + s];
+                }
+                nb[q] = bi + al * um + ga * lp;
+                nl[q] = al * lm;
+                nu[q] = ga * up;
+                nd[q] = di + al * dm + ga * dpp;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int i = 2 * p + q;
+                rl[i * S + s] = nl[q];
+                rb[i * S + s] = nb[q];
+                ru[i * S + s] = nu[q];
+                rd[i * S + s] = nd[q];
+            }
+            __syncthreads();
+        }
+    }
+    const double x0 = rd[(2 * p) * S + s] * frcp(rb[(2 * p) * S + s]);
+    const double xl = rd[(2 * p + 1) * S + s] * frcp(rb[(2 * p + 1) * S + s]);
+    __syncthreads();
+    rd[(2 * p) * S + s] = x0;  // partition end values, read by the ghost code
+    rd[(2 * p + 1) * S + s] = xl;
+    __syncthreads();  // rb (= ps) is free from here on
+    // interior rows 1..K-2 with both ends known: forward elimination with the
+    // stored pivots, back substitution with C_l = r_l c_l (results in d)
+    double *x = d;
+    {
+        double e = x0;
+#pragma unroll
+        for (int l = 1; l <= K - 2; ++l) {
+            double a, b, c;
+            coef(l, a, b, c);
+            e = rr[l] * (d[l] - a * e);
+            d[l] = e;
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        x[0] = x0;
+        x[K - 1] = xl;
+#pragma unroll
+        for (int l = K - 2; l >= 1; --l) {
+            double a, b, c;
+            coef(l, a, b, c);
+            x[l] = d[l] - rr[l] * c * x[l + 1];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    double part = 0.0;
+#pragma unroll
+    for (int l = 0; l < K; ++l) {
+        const int g = g0 + l;
+        if (g < n3m) {
+            part += x[l];
+            if (active) w[(size_t)(g + 1) * pstride] = (T)x[l];
+        }
+    }
+    // spectral z ghost planes (wall extrapolation, fft_3d_poisson.f90:809-852)
+    if (active) {
+        if (p == 0) w[0] = (T)(bc[0] * x[0] - bc[1] * x[1]);
+        const int gt = n3m - 1, pt = gt / K, lt = gt - pt * K;
+        if (p == pt) {
+            double xt = 0.0, xb = 0.0;
+#pragma unroll
+            for (int l = 0; l < K; ++l) {
+                if (l == lt) xt = x[l];
+                if (l == lt - 1) xb = x[l];
+            }
+            if (lt == 0) xb = rd[(2 * (pt - 1) + 1) * S + s];
+            w[(size_t)(n3m + 1) * pstride] = (T)(bc[2] * xt - bc[3] * xb);
+        }
+    }
+    // mean of the solution from the (0,0) real column (:628-664)
+    const bool mean_tile = (jp == z.mean_jp) && (blockIdx.x % tiles) * S <= z.mean_kx &&
+                           z.mean_kx < (blockIdx.x % tiles) * S + S;
+    if (mean_tile) {
+        ps[p * S + s] = part;
+        __syncthreads();
+        if (p == 0 && kx == z.mean_kx) {
+            double sum = 0.0;
+            for (int q = 0; q < P; ++q) sum += ps[q * S + s];
+            *mean_out = sum * z.mean_scale;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pass 3, pipelined form (the default for P <= 32 partitions).
+//
+// Same partitioned Thomas as k_zpart, with two changes that matter on
+// gfx950:
+//  - persistent workgroups walk column tiles and prefetch the next tile's K
+//    values per thread into registers before solving the current tile, so
+//    HBM traffic overlaps the arithmetic instead of every resident
+//    workgroup loading, then computing, in lockstep;
+//  - the 2P-row reduced system of each column is transposed once through LDS
+//    so that one wavefront owns a whole column (lane i = reduced row i) and
+//    cyclic reduction runs with cross-lane shuffles: two barriers per tile.
+// ---------------------------------------------------------------------------
+template <typename T, int K, int S, int WPE>
+__global__ void __launch_bounds__(S * 32) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double *__restrict__ zap,
+        double *__restrict__ mean_out, int P) {
+    constexpr int RS = 65;  // LDS column stride of the reduced system (2P <= 64, padded)
+    extern __shared__ double zsh[];
+    const int n3m = z.n3m;
+    const int NP = P * K;
+    const int R = 2 * P;
+    double *am = zsh;
+    double *ap = am + NP;
+    double *red_l = ap + NP;       // [S][RS]
+    double *red_u = red_l + S * RS;
+    double *red_d = red_u + S * RS;
+    double *ps = red_d + S * RS;   // [P][S] partial sums
+    const int tid = threadIdx.x, s = tid % S, p = tid / S;
+    const int lane = tid & 63, wv = tid >> 6, nwv = (int)((blockDim.x + 63) >> 6);
+    for (int k = tid; k < NP; k += blockDim.x) {
+        am[k] = k < n3m ? zam[k] : 0.0;
+        ap[k] = k < n3m ? zap[k] : 0.0;
+    }
+    double bc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bc[q] = zap[n3m + q];
+    const int tiles_x = (z.Lx + S - 1) / S;
+    const int ntiles = tiles_x * z.njr;
+    const size_t pstride = (size_t)z.Ly * z.Lx;
+    const int g0 = p * K;
+    const int glast = n3m - 1;
+    // Prefetch loads go to clamped, always valid addresses and nothing consumes
+    // them until the next tile starts: a select on a loaded value right after
+    // the load would make every load a full round trip (s_waitcnt per row).
+    T pre[K];
+    double pre_lx, pre_ly;
+    unsigned char pre_zx, pre_zy;
+    auto fetch = [&](int tile) {
+        const int jp = z.jlo + tile / tiles_x;
+        int kx = (tile % tiles_x) * S + s;
+        kx = kx < z.Lx ? kx : z.Lx - 1;
+        const T *w = W + (size_t)jp * z.Lx + kx;
+#pragma unroll
+        for (int l = 0; l < K; ++l) {
+            const int g = g0 + l;
+            pre[l] = w[(size_t)((g < glast ? g : glast) + 1) * pstride];
+        }
+        pre_lx = z.lamx[kx];
+        pre_ly = z.lamy[jp];
+        pre_zx = z.zx[kx];
+        pre_zy = z.zy[jp];
+    };
+    int tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile);
+    __syncthreads();
+    for (; tile < ntiles; tile += gridDim.x) {
+        const int jp = z.jlo + tile / tiles_x;
+        const int kx = (tile % tiles_x) * S + s;
+        const bool active = kx < z.Lx;
+        T *w = W + (size_t)jp * z.Lx + (active ? kx : 0);
+        const double lx = active ? pre_lx : 0.0;
+        const double ly = pre_ly;
+        const bool pin = active && pre_zx && pre_zy;
+        double d[K];
+#pragma unroll
+        for (int l = 0; l < K; ++l) d[l] = (active && g0 + l < n3m) ? (double)pre[l] : 0.0;
+        if (pin && g0 == 0) d[0] = 0.0;
+        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+        auto coef = [&](int l, double &a, double &b, double &c) {
+            const int g = g0 + l;
+            const bool id = (g >= n3m) || (pin && g == 0);
+            const double am_ = am[g], ap_ = ap[g];
+            a = am_;
+            c = id ? 0.0 : ap_;
+            b = id ? 1.0 : (-am_ - ap_ - lx) - ly;
+        };
+        double rr[K];
+        double ah, ch, dh, Af, Df, Cf;
+        {
+            double a, b, c;
+            coef(K - 2, a, b, c);
+            double r = frcp(b);
+            ah = r * a; ch = r * c; dh = r * d[K - 2];
+            coef(1, a, b, c);
+            r = frcp(b);
+            rr[1] = r;
+            Af = r * a; Cf = r * c; Df = r * d[1];
+#pragma unroll
+            for (int i = 1; i <= K - 2; ++i) {
+                const int lb = K - 2 - i, lf = 1 + i;
+                double a1, b1, c1, a2, b2, c2;
+                coef(lb, a1, b1, c1);
+                coef(lf, a2, b2, c2);
+                const double r1 = frcp(b1 - c1 * ah);
+                const double r2 = frcp(b2 - a2 * Cf);
+                ch = -r1 * c1 * ch;
+                dh = r1 * (d[lb] - c1 * dh);
+                ah = r1 * a1;
+                rr[lf] = r2;
+                Af = -r2 * a2 * Af;
+                Cf = r2 * c2;
+                Df = r2 * (d[lf] - a2 * Df);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        red_l[s * RS + 2 * p] = ah;
+        red_u[s * RS + 2 * p] = ch;
+        red_d[s * RS + 2 * p] = dh;
+        red_l[s * RS + 2 * p + 1] = Af;
+        red_u[s * RS + 2 * p + 1] = Cf;
+        red_d[s * RS + 2 * p + 1] = Df;
+        __syncthreads();
+        // cyclic reduction of each column's reduced system inside one wavefront
+        for (int col = wv; col < S; col += nwv) {
+            const bool in = lane < R;
+            double l_ = in ? red_l[col * RS + lane] : 0.0;
+            double u_ = in ? red_u[col * RS + lane] : 0.0;
+            double d_ = in ? red_d[col * RS + lane] : 0.0;
+            double b_ = 1.0;
+            for (int h = 1; h < R; h <<= 1) {
+                const int im = lane - h, ip = lane + h;
+                const double lm = __shfl(l_, im < 0 ? 0 : im), bm = __shfl(b_, im < 0 ? 0 : im);
+                const double um = __shfl(u_, im < 0 ? 0 : im), dm = __shfl(d_, im < 0 ? 0 : im);
+                const double lp = __shfl(l_, ip > 63 ? 63 : ip), bp = __shfl(b_, ip > 63 ? 63 : ip);
+                const double up = __shfl(u_, ip > 63 ? 63 : ip), dp = __shfl(d_, ip > 63 ? 63 : ip);
+                const bool hm = im >= 0, hp = ip < R;
+                const double al = hm ? -l_ * frcp(bm) : 0.0;
+                const double ga = hp ? -u_ * frcp(bp) : 0.0;
+                const double nb = b_ + al * (hm ? um : 0.0) + ga * (hp ? lp : 0.0);
+                const double nl = al * (hm ? lm : 0.0);
+                const double nu = ga * (hp ? up : 0.0);
+                const double nd = d_ + al * (hm ? dm : 0.0) + ga * (hp ? dp : 0.0);
+                b_ = nb; l_ = nl; u_ = nu; d_ = nd;
+            }
+            if (in) red_d[col * RS + lane] = d_ * frcp(b_);
+        }
+        __syncthreads();
+        const double x0 = red_d[s * RS + 2 * p];
+        const double xl = red_d[s * RS + 2 * p + 1];
+        double *x = d;
+        {
+            double e = x0;
+#pragma unroll
+            for (int l = 1; l <= K - 2; ++l) {
+                double a, b, c;
+                coef(l, a, b, c);
+                e = rr[l] * (d[l] - a * e);
+                d[l] = e;
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            x[0] = x0;
+            x[K - 1] = xl;
+#pragma unroll
+            for (int l = K - 2; l >= 1; --l) {
+                double a, b, c;
+                coef(l, a, b, c);
+                x[l] = d[l] - rr[l] * c * x[l + 1];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        double part = 0.0;
+#pragma unroll
+        for (int l = 0; l < K; ++l) {
+            const int g = g0 + l;
+            if (g < n3m) {
+                part += x[l];
+                if (active) w[(size_t)(g + 1) * pstride] = (T)x[l];
+            }
+        }
+        if (active) {
+            if (p == 0) w[0] = (T)(bc[0] * x[0] - bc[1] * x[1]);
+            const int gt = n3m - 1, pt = gt / K, lt = gt - pt * K;
+            if (p == pt) {
+                double xt = 0.0, xb = 0.0;
+#pragma unroll
+                for (int l = 0; l < K; ++l) {
+                    if (l == lt) xt = x[l];
+                    if (l == lt - 1) xb = x[l];
+                }
+                if (lt == 0) xb = red_d[s * RS + 2 * (pt - 1) + 1];
+                w[(size_t)(n3m + 1) * pstride] = (T)(bc[2] * xt - bc[3] * xb);
+            }
+        }
+        const int kx0 = (tile % tiles_x) * S;
+        if (jp == z.mean_jp && kx0 <= z.mean_kx && z.mean_kx < kx0 + S) {
+            ps[p * S + s] = part;
+            __syncthreads();
+            if (p == 0 && kx == z.mean_kx) {
+                double sum = 0.0;
+                for (int q = 0; q < P; ++q) sum += ps[q * S + s];
+                *mean_out = sum * z.mean_scale;
+            }
+        }
+        __syncthreads();  // red_* is rewritten by the next tile
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -9,8 +9,10 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pascal_poisson_fft.h"
@@ -50,9 +52,12 @@ struct ppf_plan {
     int zjlo, znjr;        // rows solved by the z pass
     int mean_kx, mean_jp;
     double mean_scale;
+    int zvariant;          // z-pass kernel shape (0 = chosen by n3m; PPF_ZVARIANT overrides)
     // device memory
     double *W = nullptr;   // spectral workspace (FP64 path)
-    double *cs = nullptr;  // Thomas scratch
+    double *cs = nullptr;  // Thomas scratch of the one-thread-per-line z kernel
+    size_t cs_elems = 0;
+    double *zcoef = nullptr;  // z row coefficients of the current solve (2*n3m+4)
     double *lamx = nullptr, *lamy = nullptr;
     unsigned char *zx = nullptr, *zy = nullptr;
     double *mean = nullptr;     // device scalar
@@ -119,17 +124,39 @@ static double eig_periodic(int m, int n, double h) {
 // ---------------------------------------------------------------------------
 #define PPF_FOR_SIZES(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
 
+static int num_cus() {
+    static int n = 0;
+    if (n <= 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+    }
+    return n;
+}
+
+// grid of a persistent kernel: as many workgroups as can be resident at once
+template <typename K>
+static unsigned persistent_grid(K kernel, int nthreads, int ntiles, size_t dyn_lds = 0) {
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, nthreads, dyn_lds) != hipSuccess || occ < 1) occ = 1;
+    const long g = (long)occ * num_cus();
+    return (unsigned)std::max(1L, std::min<long>(g, ntiles));
+}
+
 template <typename T, int N>
 static void launch_xfwd(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, hipStream_t s) {
     const int nlines = p->d.n2m * p->d.n3m;
     if (p->xkind == AX_R2C) {
         using X = XGeom<T, N, true>;
-        hipLaunchKernelGGL((k_xfwd<T, N, true>), dim3((nlines + X::LPB - 1) / X::LPB), dim3(X::NT), 0, s, in, W,
-                           nlines, p->d.n2m, p->Ly, tb.tw, tb.wk, tb.qk);
+        const unsigned g = persistent_grid(k_xfwd<T, N, true>, X::NT, (nlines + X::LPB - 1) / X::LPB);
+        hipLaunchKernelGGL((k_xfwd<T, N, true>), dim3(g), dim3(X::NT), 0, s, in, W, nlines, p->d.n2m, p->Ly, tb.tw,
+                           tb.wk, tb.qk);
     } else {
         using X = XGeom<T, N, false>;
-        hipLaunchKernelGGL((k_xfwd<T, N, false>), dim3((nlines + X::LPB - 1) / X::LPB), dim3(X::NT), 0, s, in, W,
-                           nlines, p->d.n2m, p->Ly, tb.tw, tb.wk, tb.qk);
+        const unsigned g = persistent_grid(k_xfwd<T, N, false>, X::NT, (nlines + X::LPB - 1) / X::LPB);
+        hipLaunchKernelGGL((k_xfwd<T, N, false>), dim3(g), dim3(X::NT), 0, s, in, W, nlines, p->d.n2m, p->Ly, tb.tw,
+                           tb.wk, tb.qk);
     }
 }
 
@@ -139,12 +166,14 @@ static void launch_xinv(const ppf_plan *p, const T *W, double *P, const AxisTabl
     const int nlines = p->Ly * p->Lz;
     if (p->xkind == AX_R2C) {
         using X = XGeom<T, N, true>;
-        hipLaunchKernelGGL((k_xinv<T, N, true>), dim3((nlines + X::LPB - 1) / X::LPB), dim3(X::NT), 0, s, W, P,
-                           nlines, tb.tw, tb.wk, tb.qk, dmx1, p->mean);
+        const unsigned g = persistent_grid(k_xinv<T, N, true>, X::NT, (nlines + X::LPB - 1) / X::LPB);
+        hipLaunchKernelGGL((k_xinv<T, N, true>), dim3(g), dim3(X::NT), 0, s, W, P, nlines, tb.tw, tb.wk, tb.qk, dmx1,
+                           p->mean);
     } else {
         using X = XGeom<T, N, false>;
-        hipLaunchKernelGGL((k_xinv<T, N, false>), dim3((nlines + X::LPB - 1) / X::LPB), dim3(X::NT), 0, s, W, P,
-                           nlines, tb.tw, tb.wk, tb.qk, dmx1, p->mean);
+        const unsigned g = persistent_grid(k_xinv<T, N, false>, X::NT, (nlines + X::LPB - 1) / X::LPB);
+        hipLaunchKernelGGL((k_xinv<T, N, false>), dim3(g), dim3(X::NT), 0, s, W, P, nlines, tb.tw, tb.wk, tb.qk, dmx1,
+                           p->mean);
     }
 }
 
@@ -153,12 +182,15 @@ static void launch_y_mode(const ppf_plan *p, T *W, const AxisTables<T> &tb, bool
                           hipStream_t s) {
     using Y = YGeom<T, N, MODE>;
     const int nb = (p->Lx + Y::TX - 1) / Y::TX;
-    if (!inverse)
-        hipLaunchKernelGGL((k_yfwd<T, N, MODE>), dim3(nb, p->d.n3m), dim3(Y::NT), 0, s, W, p->Lx, p->Ly, tb.tw, tb.wk,
+    if (!inverse) {
+        const unsigned g = persistent_grid(k_yfwd<T, N, MODE>, Y::NT, nb * p->d.n3m);
+        hipLaunchKernelGGL((k_yfwd<T, N, MODE>), dim3(g), dim3(Y::NT), 0, s, W, p->Lx, p->Ly, p->d.n3m, tb.tw, tb.wk,
                            tb.qk);
-    else
-        hipLaunchKernelGGL((k_yinv<T, N, MODE>), dim3(nb, p->Lz), dim3(Y::NT), 0, s, W, p->Lx, p->Ly, tb.tw, tb.wk,
+    } else {
+        const unsigned g = persistent_grid(k_yinv<T, N, MODE>, Y::NT, nb * p->Lz);
+        hipLaunchKernelGGL((k_yinv<T, N, MODE>), dim3(g), dim3(Y::NT), 0, s, W, p->Lx, p->Ly, p->Lz, tb.tw, tb.wk,
                            tb.qk, dmx2);
+    }
 }
 
 template <typename T, int N>
@@ -217,17 +249,70 @@ static int launch_z(ppf_plan *p, double *W, const double *dx3, const double *dmx
     z.mean_jp = p->mean_jp;
     z.mean_scale = p->mean_scale;
     const int n3m = p->d.n3m;
-    const long ncol = (long)p->Lx * p->znjr;
-    const int nt = 256;
-    const size_t shm = 2 * sizeof(double) * (size_t)n3m;
-    hipLaunchKernelGGL((k_zthomas<double>), dim3((unsigned)((ncol + nt - 1) / nt)), dim3(nt), shm, s, W, p->cs, z, dx3,
-                       dmx3, p->mean);
+    // partitioned register-resident Thomas, shape (K rows per thread, S
+    // columns per workgroup); launched only when S*P <= NTMAX and the LDS
+    // footprint stays within 64 KiB, else the one-thread-per-line kernel
+    bool done = false;
+    auto part = [&](auto kk, auto ss, auto nt, auto wpe) {
+        constexpr int K = decltype(kk)::value, S = decltype(ss)::value;
+        constexpr int NTMAX = decltype(nt)::value, WPE = decltype(wpe)::value;
+        const int P = (n3m + K - 1) / K;
+        const size_t shm = sizeof(double) * (2 * (size_t)P * K + 8 * (size_t)P * S);
+        if (P < 1 || S * P > NTMAX || shm > 65536) return;
+        const int tiles = (p->Lx + S - 1) / S;
+        double *zam = p->zcoef, *zap = p->zcoef + n3m;
+        hipLaunchKernelGGL(k_zcoef, dim3((n3m + 255) / 256), dim3(256), 0, s, dx3, dmx3, n3m, zam, zap);
+        hipLaunchKernelGGL((k_zpart<double, K, S, NTMAX, WPE>), dim3((unsigned)(tiles * p->znjr)), dim3(S * P), shm,
+                           s, W, z, zam, zap, dmx3, p->mean, P);
+        done = true;
+    };
+    // pipelined variant: persistent, prefetching, wavefront cyclic reduction
+    auto pipe = [&](auto kk, auto ss, auto wpe) {
+        constexpr int K = decltype(kk)::value, S = decltype(ss)::value, WPE = decltype(wpe)::value;
+        const int P = (n3m + K - 1) / K;
+        const size_t shm = sizeof(double) * (2 * (size_t)P * K + 3 * (size_t)S * 65 + (size_t)P * S);
+        if (P < 1 || P > 32 || K < 2 || shm > 65536) return;
+        double *zam = p->zcoef, *zap = p->zcoef + n3m;
+        hipLaunchKernelGGL(k_zcoef, dim3((n3m + 255) / 256), dim3(256), 0, s, dx3, dmx3, n3m, zam, zap);
+        const int ntiles = ((p->Lx + S - 1) / S) * p->znjr;
+        const unsigned g = persistent_grid(k_zpipe<double, K, S, WPE>, S * P, ntiles, shm);
+        hipLaunchKernelGGL((k_zpipe<double, K, S, WPE>), dim3(g), dim3(S * P), shm, s, W, z, zam, zap, p->mean, P);
+        done = true;
+    };
+    using std::integral_constant;
+    int v = p->zvariant;
+    if (v == 0) v = (n3m <= 512) ? 7 : (n3m <= 1024 ? 3 : 4);
+    switch (v) {
+        case 7: pipe(integral_constant<int, 16>{}, integral_constant<int, 16>{}, integral_constant<int, 2>{}); break;
+        case 8: pipe(integral_constant<int, 16>{}, integral_constant<int, 8>{}, integral_constant<int, 2>{}); break;
+        case 9: pipe(integral_constant<int, 32>{}, integral_constant<int, 8>{}, integral_constant<int, 2>{}); break;
+        case 10: pipe(integral_constant<int, 32>{}, integral_constant<int, 16>{}, integral_constant<int, 2>{}); break;
+        case 11: pipe(integral_constant<int, 8>{}, integral_constant<int, 16>{}, integral_constant<int, 4>{}); break;
+        default: break;
+    }
+    switch (v) {
+        case 1: part(integral_constant<int, 16>{}, integral_constant<int, 16>{}, integral_constant<int, 512>{}, integral_constant<int, 3>{}); break;
+        case 2: part(integral_constant<int, 16>{}, integral_constant<int, 8>{}, integral_constant<int, 512>{}, integral_constant<int, 3>{}); break;
+        case 3: part(integral_constant<int, 32>{}, integral_constant<int, 16>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
+        case 4: part(integral_constant<int, 32>{}, integral_constant<int, 8>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
+        case 5: part(integral_constant<int, 8>{}, integral_constant<int, 8>{}, integral_constant<int, 512>{}, integral_constant<int, 4>{}); break;
+        default: break;
+    }
+    if (!done) {
+        const long ncol = (long)p->Lx * p->znjr;
+        // the one-thread-per-line kernel keeps its forward multipliers in cs
+        if (p->cs_elems < (size_t)ncol * (size_t)n3m) return PPF_ERR_UNSUPPORTED;
+        const int nt = 256;
+        const size_t shm = 2 * sizeof(double) * (size_t)n3m;
+        hipLaunchKernelGGL((k_zthomas<double>), dim3((unsigned)((ncol + nt - 1) / nt)), dim3(nt), shm, s, W, p->cs, z,
+                           dx3, dmx3, p->mean);
+    }
     return PPF_OK;
 }
 
 static void free_plan(ppf_plan *p) {
     if (!p) return;
-    void *ptrs[] = {p->W, p->cs, p->lamx, p->lamy, p->zx, p->zy, p->mean, p->partial, p->red,
+    void *ptrs[] = {p->W, p->cs, p->zcoef, p->lamx, p->lamy, p->zx, p->zy, p->mean, p->partial, p->red,
                     p->tx64.tw, p->tx64.wk, p->tx64.qk, p->ty64.tw, p->ty64.wk, p->ty64.qk};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
@@ -280,15 +365,22 @@ int ppf_plan_create(const ppf_desc *desc, ppf_plan **out) {
     else { p->zjlo = 1; p->znjr = d.n2m; }
     p->mean_kx = 0;
     p->mean_jp = (p->ymode == 2) ? 0 : 1;
+    p->zvariant = 0;
+    if (const char *zv = std::getenv("PPF_ZVARIANT")) p->zvariant = std::atoi(zv);
     const double fx = px ? 1.0 : 2.0, fy = (p->ymode == 0) ? 2.0 : 1.0;
     p->mean_scale = 1.0 / (fx * fy * (double)d.n1m * (double)d.n2m * (double)d.n3m);
 
     int rc = PPF_OK;
     const size_t wsz = (size_t)p->Lx * p->Ly * p->Lz;
-    const size_t csz = (size_t)p->Lx * p->znjr * d.n3m;
+    // Thomas scratch only where the one-thread-per-line kernel can run: long
+    // lines (n3m > 2048) or when it is selected explicitly
+    const bool need_cs = d.n3m > 2048 || p->zvariant == 6;
+    const size_t csz = need_cs ? (size_t)p->Lx * p->znjr * d.n3m : 1;
+    p->cs_elems = csz;
     if ((rc = dev_alloc(p, &p->W, wsz)) || (rc = dev_alloc(p, &p->cs, csz)) || (rc = dev_alloc(p, &p->lamx, p->Lx)) ||
         (rc = dev_alloc(p, &p->lamy, p->Ly)) || (rc = dev_alloc(p, &p->zx, p->Lx)) || (rc = dev_alloc(p, &p->zy, p->Ly)) ||
-        (rc = dev_alloc(p, &p->mean, 1)) || (rc = dev_alloc(p, &p->red, 2))) {
+        (rc = dev_alloc(p, &p->mean, 1)) || (rc = dev_alloc(p, &p->red, 2)) ||
+        (rc = dev_alloc(p, &p->zcoef, 2 * (size_t)d.n3m + 4))) {
         free_plan(p);
         return rc;
     }
