@@ -6,12 +6,14 @@ Workload: 512^3 cells, FP64, Neumann x and y (DCT-II/III), tanh-stretched z
 one 3x3x3 box mean with the dx3-weighted mean removed (synthetic, seed 2).
 A step is one full solve (RHS in HBM -> ghosted solution in HBM).
 
-Prints one JSON line: throughput in Mpts/s (cells / solve time), the per-pass
-HBM roofline of the dominant pass (algorithmic bytes / pass time from HIP
-events on the solve stream), and the FP64 CPU oracle timed on a slab of the
-same workload on the host cores as a reported baseline.
+Prints one JSON line: throughput in Mpts/s (cells / FP64 solve time), the
+per-pass HBM roofline of the dominant pass (algorithmic bytes / pass time from
+HIP events on the solve stream), the FP32+refinement solve to a 1e-10 relative
+residual against the FP64 solve plus its certification residual, and the FP64
+CPU oracle timed on a slab of the same workload on the host cores as a
+reported baseline.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--n 512] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--n 512] [--no-cpu] [--no-ir]
 """
 from __future__ import annotations
 
@@ -62,6 +64,39 @@ def cpu_baseline(n: int, nz_sample: int, reps: int = 5):
     }
 
 
+def precision_leg(api, prob, dev, n, steps, fp64_ms, res_ms, fp64_res):
+    """FP32 transforms + FP64 refinement to ||f - Lx||/||f|| <= 1e-10, against
+    the FP64 solve plus its certification residual, on the same inputs."""
+    solve_args = (dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3)
+    p64 = dev.p.download()
+    plan = api.FFTPoisson(n, n, n, prob.L, prob.pbc, precision=api.PPF_PREC_FP32_IR, tol=1e-10, max_sweeps=8)
+    plan.solve(*solve_args)
+    api.sync()
+    reps = max(3, steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = plan.solve(*solve_args)
+    api.sync()
+    ir_ms = (time.perf_counter() - t0) / reps * 1e3
+    stage32 = np.array([plan.solve_timed(*solve_args) for _ in range(3)]).mean(axis=0)
+    st = plan.solve(*solve_args)
+    api.sync()
+    p32 = dev.p.download()
+    err = float(np.linalg.norm(p32 - p64) / np.linalg.norm(p64))
+    plan.clean()
+    print(f"# fp32+IR: {ir_ms:.3f} ms, {st.sweeps} sweeps, residual {st.rel_residual:.2e}", file=sys.stderr,
+          flush=True)
+    return {
+        "target_rel_residual": 1e-10,
+        "fp64": {"solve_ms": round(fp64_ms, 4), "certify_residual_ms": round(res_ms, 4),
+                 "to_target_ms": round(fp64_ms + res_ms, 4), "rel_residual": fp64_res},
+        "fp32_ir": {"to_target_ms": round(ir_ms, 4), "sweeps": int(st.sweeps), "rel_residual": st.rel_residual,
+                    "rel_l2_vs_fp64_solution": err,
+                    "fp32_solve_stage_ms": {k: round(float(v), 4) for k, v in zip(api.FFTPoisson.STAGES, stage32)}},
+        "fp32_ir_speedup_vs_fp64": round((fp64_ms + res_ms) / ir_ms, 3),
+    }
+
+
 def load_traffic(kernel_key: str):
     """HBM bytes per launch of the dominant pass from committed rocprofv3 PMC
     summaries (profiles/traffic.json), or None."""
@@ -82,6 +117,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-ir", action="store_true", help="skip the FP32+refinement comparison")
     ap.add_argument("--cpu-nz", type=int, default=0, help="z extent of the CPU baseline slab (default n/4)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
@@ -119,7 +155,19 @@ def main(argv=None):
     stages = np.array([plan.solve_timed(*solve_args) for _ in range(max(3, a.steps))])
     stage_ms = stages.mean(axis=0)
     rel_res = plan.residual(dev.rhs, dev.p, dev.dmx3, dev.dx3)
+    # FP64 certification residual (the 1e-10 check the FP64 path needs too)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        plan.residual(dev.rhs, dev.p, dev.dmx3, dev.dx3)
+    res_ms = (time.perf_counter() - t0) / 3 * 1e3
     npts = n ** 3
+    plan.clean()
+    prec = None
+    if not a.no_ir:
+        prec = precision_leg(api, prob, dev, n, a.steps, ms, res_ms, rel_res)
+    dev.free()
+    # plain copy kernel over one pass's worth of data: measured-bandwidth yardstick
+    copy_gbs = api.copy_bandwidth(npts, reps=max(3, a.steps // 2))
     dom = int(np.argmax(stage_ms))
     dom_name = api.FFTPoisson.STAGES[dom]
     achieved = BYTES_PER_PT_STAGE * npts / (stage_ms[dom] * 1e-3) / 1e9
@@ -154,12 +202,12 @@ def main(argv=None):
             "traffic": traffic,
             "algorithmic_bytes_per_launch": BYTES_PER_PT_STAGE * npts,
         },
+        "copy_kernel_gbs": round(copy_gbs, 1),
         "solve_roofline_frac": round(BYTES_PER_PT_SOLVE * npts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "stage_ms": {k: round(float(v), 4) for k, v in zip(api.FFTPoisson.STAGES, stage_ms)},
         "rel_residual": rel_res,
+        "fp32_refine_vs_fp64": prec,
     }
-    dev.free()
-    plan.clean()
     if not a.no_cpu:
         nz = a.cpu_nz or max(2, n // 4)
         print(f"# cpu baseline on {n}x{n}x{nz}", file=sys.stderr, flush=True)

@@ -48,8 +48,9 @@ typedef struct ppf_desc {
     double L1, L2, L3;    /* domain lengths (x/y spacing L/nm is uniform) */
     int pbc1, pbc2, pbc3; /* nonzero: periodic; zero: Neumann walls */
     int precision;        /* PPF_PREC_FP64 or PPF_PREC_FP32_IR */
-    double tol;           /* FP32_IR: target ||f-Lp||/||f|| (<=0: 1e-10) */
-    int max_sweeps;       /* FP32_IR: refinement sweeps cap (<=0: 8) */
+    double tol;           /* FP32_IR: target ||f-Lp||/||f|| (0: 1e-10; < 0:
+                             run exactly max_sweeps corrections, no test) */
+    int max_sweeps;       /* FP32_IR: cap on correction sweeps (<=0: 8) */
 } ppf_desc;
 
 typedef struct ppf_stats {
@@ -75,15 +76,20 @@ int ppf_plan_create(const ppf_desc *desc, ppf_plan **plan);
  *                                      reference's Fortran element q (0:nsub)
  *   stream  hipStream_t or NULL
  *   stats   may be NULL
- * FP64 plans run asynchronously.  FP32_IR plans with tol > 0 synchronise once
- * per sweep to test convergence; with tol < 0 they run exactly max_sweeps
- * sweeps without synchronising. */
+ * FP64 plans run asynchronously.  FP32_IR plans: an FP32 solve, then
+ * refinement sweeps x += solve32(f - L x) with the residual and the update in
+ * FP64.  With tol >= 0 they synchronise once per sweep to test
+ * ||f - L x||/||f|| <= tol (stats gets the sweeps and that residual;
+ * PPF_ERR_NOT_CONVERGED after max_sweeps, p then holds the last iterate); with
+ * tol < 0 they run exactly max_sweeps sweeps without synchronising (and
+ * measure the residual only when stats is given). */
 int ppf_solve(ppf_plan *plan, const double *prhs, double *p, const double *dmx1, const double *dmx2,
               const double *dmx3, const double *dx3, void *stream, ppf_stats *stats);
 
 /* ppf_solve with HIP events recorded on `stream` around each pass; on return
  * (the stream is synchronised) stage_ms[0..4] hold the durations of the
- * x-forward, y-forward, z-Thomas, y-inverse and x-inverse passes.  The
+ * x-forward, y-forward, z-Thomas, y-inverse and x-inverse passes (for an
+ * FP32_IR plan: of one FP32 solve, no refinement).  The
  * counterpart of the reference's NVTX ranges Poisson-F_x, -F_y, -TDMA-z,
  * -B_y, -B_x (fft_3d_poisson.f90:377-624), for measurement. */
 int ppf_solve_timed(ppf_plan *plan, const double *prhs, double *p, const double *dmx1, const double *dmx2,
@@ -133,6 +139,14 @@ int ppf_memset(void *ptr, int value, size_t bytes);
 int ppf_sync(void *stream);
 int ppf_device_count(int *count);
 int ppf_set_device(int device);
+
+/* ---- measurement helper (no counterpart in the reference) --------------- */
+/* Streams `count` doubles (count even, 16-B aligned buffers) from src to dst
+ * `reps` times with a plain copy kernel on `stream` and returns the mean
+ * duration of one copy in *ms (HIP events; synchronises).  The measured
+ * bandwidth 16*count/ms is the yardstick the pass kernels are compared with,
+ * and the known byte count calibrates rocprofv3's HBM byte counters. */
+int ppf_copy_timed(double *dst, const double *src, size_t count, int reps, void *stream, float *ms);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,7 @@ LIB_PATH = os.path.join(_PKG, "lib", "libpascal_poisson_fft.so")
 
 PPF_PREC_FP64 = 0
 PPF_PREC_FP32_IR = 1
+PPF_ERR_NOT_CONVERGED = -6
 
 H2D, D2H, D2D = 1, 2, 3
 
@@ -80,6 +81,7 @@ def lib() -> C.CDLL:
     L.ppf_sync.argtypes = [vp]
     L.ppf_device_count.argtypes = [C.POINTER(I)]
     L.ppf_set_device.argtypes = [I]
+    L.ppf_copy_timed.argtypes = [vp, vp, sz, I, vp, C.POINTER(C.c_float)]
     _lib = L
     return L
 
@@ -101,6 +103,21 @@ def device_count() -> int:
 
 def sync(stream=None) -> None:
     _check(lib().ppf_sync(stream), "ppf_sync")
+
+
+def copy_bandwidth(count: int = 1 << 28, reps: int = 10) -> float:
+    """GB/s of the library's plain device copy kernel over `count` doubles
+    (read + write bytes / time)."""
+    a, b = DeviceArray(count), DeviceArray(count)
+    try:
+        a.fill_zero()
+        ms = C.c_float(0.0)
+        _check(lib().ppf_copy_timed(b.ptr, a.ptr, count, 1, None, C.byref(ms)), "ppf_copy_timed")
+        _check(lib().ppf_copy_timed(b.ptr, a.ptr, count, reps, None, C.byref(ms)), "ppf_copy_timed")
+        return 16.0 * count / (ms.value * 1e-3) / 1e9
+    finally:
+        a.free()
+        b.free()
 
 
 class DeviceArray:
@@ -167,10 +184,14 @@ class FFTPoisson:
         return int(lib().ppf_plan_workspace_bytes(self.handle))
 
     def solve(self, prhs: DeviceArray, p: DeviceArray, dmx1: DeviceArray, dmx2: DeviceArray, dmx3: DeviceArray,
-              dx3: DeviceArray, stream=None) -> Stats:
+              dx3: DeviceArray, stream=None, allow_not_converged: bool = False) -> Stats:
+        """One solve; returns its Stats (FP32_IR: sweeps and final relative
+        residual).  A refinement that stops above tol raises, unless
+        allow_not_converged (p then holds the last iterate)."""
         st = Stats()
-        _check(lib().ppf_solve(self.handle, prhs.ptr, p.ptr, dmx1.ptr, dmx2.ptr, dmx3.ptr, dx3.ptr, stream,
-                               C.byref(st)), "ppf_solve")
+        rc = lib().ppf_solve(self.handle, prhs.ptr, p.ptr, dmx1.ptr, dmx2.ptr, dmx3.ptr, dx3.ptr, stream, C.byref(st))
+        if not (rc == PPF_ERR_NOT_CONVERGED and allow_not_converged):
+            _check(rc, "ppf_solve")
         return st
 
     STAGES = ("x-forward", "y-forward", "z-thomas", "y-inverse", "x-inverse")

@@ -62,15 +62,80 @@ __device__ __forceinline__ int fdiv_small(int a, int b, float rb) {
     return q;
 }
 
+// Residual input of the refinement form of pass 1: r = f - L x on interior
+// cells, x the ghosted FP64 iterate.  L is the operator the solve inverts:
+// uniform x/y spacing with zero-flux walls (the DCT eigenvalues) or wrap, and
+// the stretched z rows zam/zap of fft_3d_poisson.f90:1143-1146.
+struct XRes {
+    const double *P;          // (n1m+2)(n2m+2)(n3m+2) iterate
+    const double *zam, *zap;  // z rows (zero at the walls)
+    double *partial;          // per-workgroup sums of r^2 and f^2
+    double ih1, ih2;          // 1/h^2
+    int pbc1, pbc2;
+};
+
+// r at interior cell (n, j, k) (0-based); fv receives f there.  Neighbour
+// addresses are selected, never loaded values, so the seven loads issue
+// together.
+__device__ __forceinline__ double residual_at(const double *__restrict__ f, const XRes &r, int n, int j, int k, int N,
+                                              int n2m, double &fv) {
+    const size_t L1 = (size_t)N + 2;
+    const size_t L2 = (size_t)n2m + 2;
+    const size_t pc = ((size_t)(k + 1) * L2 + (size_t)(j + 1)) * L1 + (size_t)(n + 1);
+    const size_t pl = (size_t)L1 * L2;
+    size_t ixm = pc - 1, ixp = pc + 1, iym = pc - L1, iyp = pc + L1;
+    if (r.pbc1) {
+        ixm = n == 0 ? pc + (N - 1) : ixm;
+        ixp = n == N - 1 ? pc - (N - 1) : ixp;
+    } else {
+        ixm = n == 0 ? pc : ixm;
+        ixp = n == N - 1 ? pc : ixp;
+    }
+    if (r.pbc2) {
+        iym = j == 0 ? pc + (size_t)(n2m - 1) * L1 : iym;
+        iyp = j == n2m - 1 ? pc - (size_t)(n2m - 1) * L1 : iyp;
+    } else {
+        iym = j == 0 ? pc : iym;
+        iyp = j == n2m - 1 ? pc : iyp;
+    }
+    const double c = r.P[pc], xm = r.P[ixm], xp = r.P[ixp], ym = r.P[iym], yp = r.P[iyp];
+    const double zm = r.P[pc - pl], zp = r.P[pc + pl];  // ghost planes at the walls (finite, weight 0)
+    const double am = r.zam[k], ap = r.zap[k];
+    fv = f[((size_t)k * n2m + j) * N + n];
+    const double lp = (xp - 2.0 * c + xm) * r.ih1 + (yp - 2.0 * c + ym) * r.ih2 + am * zm - (am + ap) * c + ap * zp;
+    return fv - lp;
+}
+
+// Sum of v over a workgroup of NT <= 256 threads (any count); thread 0 gets
+// the total.  red holds NT doubles of LDS.  Fixed order: deterministic.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+#pragma unroll
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h && tid + h < NT) red[tid] += red[tid + h];
+        __syncthreads();
+    }
+    const double t = red[0];
+    __syncthreads();
+    return t;
+}
+
 // The line-transform kernels are persistent and software-pipelined: a
 // workgroup walks tiles tile = blockIdx.x, blockIdx.x + gridDim.x, ...; the
 // next tile's global loads are issued into registers right after the current
 // tile has been scattered into LDS, so they are in flight while the current
 // tile is transformed and stored (barriers wait on LDS only).
-template <typename T, int N, bool PER>
+//
+// RES: the refinement form -- the transformed line is the FP64 residual
+// r = f - L x (in = f, xr.P = x) rather than in itself, and the workgroup's
+// sums of r^2 and f^2 go to xr.partial.
+template <typename T, int N, bool PER, bool RES = false>
 __global__ void __launch_bounds__(256)
 k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, int Ly,
-       const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk) {
+       const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk, XRes xr) {
     using X = XGeom<T, N, PER>;
     using G = typename X::G;
     using L = typename X::L;
@@ -82,18 +147,30 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
     const int ntiles = (nlines + LPB - 1) / LPB;
     const float rn2m = 1.0f / (float)n2m;
     double pre[EPT];
+    double sr = 0.0, sf = 0.0;
     // unconditional loads from clamped addresses (lines past the end are
     // transformed but never stored); nothing consumes them before the next
     // tile, so they stay in flight
     auto fetch = [&](int tile) {
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
-            int e = tid + q * NT;
-            e = e < LPB * N ? e : LPB * N - 1;
+            const int e0 = tid + q * NT;
+            const int e = e0 < LPB * N ? e0 : LPB * N - 1;
             const int li = e / N, n = e - li * N;
-            int line = tile * LPB + li;
-            line = line < nlines ? line : nlines - 1;
-            pre[q] = in[(size_t)line * N + n];
+            const int line0 = tile * LPB + li;
+            const int line = line0 < nlines ? line0 : nlines - 1;
+            if constexpr (RES) {
+                const int k = fdiv_small(line, n2m, rn2m), j = line - k * n2m;
+                double fv;
+                const double r = residual_at(in, xr, n, j, k, N, n2m, fv);
+                pre[q] = r;
+                if (e0 < LPB * N && line0 < nlines) {
+                    sr += r * r;
+                    sf += fv * fv;
+                }
+            } else {
+                pre[q] = in[(size_t)line * N + n];
+            }
         }
     };
     int tile = blockIdx.x;
@@ -128,13 +205,24 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
             }
         }
     }
+    if constexpr (RES) {
+        __shared__ double red[NT];
+        const double tr = block_sum<NT>(sr, red);
+        const double tf = block_sum<NT>(sf, red);
+        if (tid == 0) {
+            xr.partial[2 * blockIdx.x] = tr;
+            xr.partial[2 * blockIdx.x + 1] = tf;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
 // pass 5: x-inverse.  Every W row (Ly*Lz of them) -> one P row of N+2 with the
 // x ghosts; subtracts the mean found by the z pass.
+// ACC: the refinement form -- the row is added to P (x += e) instead of
+// stored; P's rows are prefetched with the W rows.
 // ---------------------------------------------------------------------------
-template <typename T, int N, bool PER>
+template <typename T, int N, bool PER, bool ACC = false>
 __global__ void __launch_bounds__(256)
 k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk,
@@ -146,10 +234,12 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
     constexpr int LX = PER ? N + 2 : N;
     constexpr int LP = N + 2;
     constexpr int EPT = (LPB * LX + NT - 1) / NT;
+    constexpr int EPTP = (LPB * LP + NT - 1) / NT;  // P elements stored per thread per tile
     __shared__ T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int ntiles = (nlines + LPB - 1) / LPB;
     T pre[EPT];
+    double pold[ACC ? EPTP : 1];
     auto fetch = [&](int tile) {
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
@@ -158,6 +248,16 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
             size_t idx = (size_t)tile * LPB * LX + e;
             const size_t last = (size_t)nlines * LX - 1;
             pre[q] = W[idx < last ? idx : last];
+        }
+        if constexpr (ACC) {
+#pragma unroll
+            for (int q = 0; q < EPTP; ++q) {
+                int e = tid + q * NT;
+                e = e < LPB * LP ? e : LPB * LP - 1;
+                const size_t idx = (size_t)tile * LPB * LP + e;
+                const size_t last = (size_t)nlines * LP - 1;
+                pold[q] = P[idx < last ? idx : last];
+            }
         }
     };
     int tile = blockIdx.x;
@@ -178,42 +278,51 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
     const cx<T> *tab = L::tables(lds);
     cx<T> *my = L::line(lds, l);
     for (; tile < ntiles; tile += gridDim.x) {
-    __syncthreads();  // the previous tile's LDS lines have been stored
+        __syncthreads();  // the previous tile's LDS lines have been stored
 #pragma unroll
-    for (int q = 0; q < EPT; ++q) {
-        const int e = tid + q * NT;
-        if (e < LPB * LX) {
-            const int li = e / LX, n = e - li * LX;
-            L::real(lds, li)[n] = pre[q];
+        for (int q = 0; q < EPT; ++q) {
+            const int e = tid + q * NT;
+            if (e < LPB * LX) {
+                const int li = e / LX, n = e - li * LX;
+                L::real(lds, li)[n] = pre[q];
+            }
         }
-    }
-    if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
-    __syncthreads();
-    if (PER)
-        c2r_pre<T, N, TPL>(my, t, tab + M);
-    else
-        idct_pre<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
-    fft_forward<T, M, TPL>(my, t, tab);
-    for (int e = tid; e < LPB * LP; e += NT) {
-        const int li = e / LP, np = e - li * LP;
-        const int line = tile * LPB + li;
-        if (line >= nlines) continue;
-        const T *re = L::real(lds, li);
-        // physical sample n (0-based) of this line, mean removed
-        auto sample = [&](int n) -> double {
-            const int tpos = PER ? n : makhoul_pos<N>(n);
-            const double v = (double)re[tpos];
-            return ((tpos & 1) ? -v : v) * scale - mu;
-        };
-        double out;
-        if (np == 0)
-            out = PER ? sample(N - 1) : bca0 * sample(0) - bcb0 * sample(1);
-        else if (np == N + 1)
-            out = PER ? sample(0) : bca1 * sample(N - 1) - bcb1 * sample(N - 2);
+        double pcur[ACC ? EPTP : 1];
+        if constexpr (ACC) {
+#pragma unroll
+            for (int q = 0; q < EPTP; ++q) pcur[q] = pold[q];
+        }
+        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+        __syncthreads();
+        if (PER)
+            c2r_pre<T, N, TPL>(my, t, tab + M);
         else
-            out = sample(np - 1);
-        P[(size_t)line * LP + np] = out;
-    }
+            idct_pre<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
+        fft_forward<T, M, TPL>(my, t, tab);
+#pragma unroll
+        for (int q = 0; q < EPTP; ++q) {
+            const int e = tid + q * NT;
+            if (e >= LPB * LP) break;
+            const int li = e / LP, np = e - li * LP;
+            const int line = tile * LPB + li;
+            if (line >= nlines) continue;
+            const T *re = L::real(lds, li);
+            // physical sample n (0-based) of this line, mean removed
+            auto sample = [&](int n) -> double {
+                const int tpos = PER ? n : makhoul_pos<N>(n);
+                const double v = (double)re[tpos];
+                return ((tpos & 1) ? -v : v) * scale - mu;
+            };
+            double out;
+            if (np == 0)
+                out = PER ? sample(N - 1) : bca0 * sample(0) - bcb0 * sample(1);
+            else if (np == N + 1)
+                out = PER ? sample(0) : bca1 * sample(N - 1) - bcb1 * sample(N - 2);
+            else
+                out = sample(np - 1);
+            if constexpr (ACC) out += pcur[q];
+            P[(size_t)line * LP + np] = out;
+        }
     }
 }
 
@@ -757,7 +866,8 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
 }
 
 // ---------------------------------------------------------------------------
-// pass 3, pipelined form (the default for P <= 32 partitions).
+// pass 3, pipelined form (an alternative shape; the default is k_zpart with
+// K = 32, S = 16, which measured faster at 512^3 on MI355X).
 //
 // Same partitioned Thomas as k_zpart, with two changes that matter on
 // gfx950:
@@ -965,69 +1075,30 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
 }
 
 // ---------------------------------------------------------------------------
-// FP64 residual r = f - L p on interior cells of the ghosted P, with partial
-// sums of r^2 and f^2 per workgroup (deterministic two-level reduction).
+// FP64 residual r = f - L p on interior cells of the ghosted P (the operator
+// of residual_at), with per-workgroup sums of r^2 and f^2 (deterministic
+// two-level reduction with k_reduce_pairs).  Workgroups stride over the
+// n2m*n3m rows; threads over a row.
 // ---------------------------------------------------------------------------
-struct ResArgs {
-    int n1m, n2m, n3m, pbc1, pbc2;
-    double ih1, ih2;  // 1/h^2
-};
-
 __global__ void __launch_bounds__(256)
-k_residual(const double *__restrict__ f, const double *__restrict__ P, ResArgs a, const double *__restrict__ dx3,
-           const double *__restrict__ dmx3, double *__restrict__ rout, double *__restrict__ partial) {
-    __shared__ double s_r[256], s_f[256];
-    const int n1m = a.n1m, n2m = a.n2m, n3m = a.n3m;
-    const size_t N = (size_t)n1m * n2m * n3m;
-    const int L1 = n1m + 2, L2 = n2m + 2;
+k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m) {
+    __shared__ double red[256];
     double sr = 0.0, sf = 0.0;
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(q % n1m);
-        const size_t rest = q / n1m;
-        const int j = (int)(rest % n2m), k = (int)(rest / n2m);
-        const size_t pc = ((size_t)(k + 1) * L2 + (j + 1)) * L1 + (i + 1);
-        const double c = P[pc];
-        double xm, xp, ym, yp;
-        if (a.pbc1) {
-            xm = P[pc - 1 + (i == 0 ? n1m : 0)];
-            xp = P[pc + 1 - (i == n1m - 1 ? n1m : 0)];
-        } else {
-            xm = i == 0 ? c : P[pc - 1];
-            xp = i == n1m - 1 ? c : P[pc + 1];
+    const int nrows = n2m * n3m;
+    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+        const int k = row / n2m, j = row - k * n2m;
+        for (int i = threadIdx.x; i < n1m; i += blockDim.x) {
+            double fv;
+            const double r = residual_at(f, xr, i, j, k, n1m, n2m, fv);
+            sr += r * r;
+            sf += fv * fv;
         }
-        if (a.pbc2) {
-            ym = P[pc - L1 + (j == 0 ? (size_t)n2m * L1 : 0)];
-            yp = P[pc + L1 - (j == n2m - 1 ? (size_t)n2m * L1 : 0)];
-        } else {
-            ym = j == 0 ? c : P[pc - L1];
-            yp = j == n2m - 1 ? c : P[pc + L1];
-        }
-        const size_t pl = (size_t)L1 * L2;
-        const int kk = k + 1;
-        const double amk = kk == 1 ? 0.0 : 1.0 / dx3[kk] / dmx3[kk];
-        const double apk = kk == n3m ? 0.0 : 1.0 / dx3[kk] / dmx3[kk + 1];
-        const double zm = kk == 1 ? 0.0 : P[pc - pl];
-        const double zp = kk == n3m ? 0.0 : P[pc + pl];
-        const double lp = (xp - 2.0 * c + xm) * a.ih1 + (yp - 2.0 * c + ym) * a.ih2 + amk * zm - (amk + apk) * c + apk * zp;
-        const double fv = f[q];
-        const double r = fv - lp;
-        if (rout) rout[q] = r;
-        sr += r * r;
-        sf += fv * fv;
     }
-    s_r[threadIdx.x] = sr;
-    s_f[threadIdx.x] = sf;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            s_r[threadIdx.x] += s_r[threadIdx.x + s];
-            s_f[threadIdx.x] += s_f[threadIdx.x + s];
-        }
-        __syncthreads();
-    }
+    const double tr = block_sum<256>(sr, red);
+    const double tf = block_sum<256>(sf, red);
     if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = s_r[0];
-        partial[2 * blockIdx.x + 1] = s_f[0];
+        xr.partial[2 * blockIdx.x] = tr;
+        xr.partial[2 * blockIdx.x + 1] = tf;
     }
 }
 
@@ -1052,6 +1123,13 @@ __global__ void __launch_bounds__(256) k_reduce_pairs(const double *__restrict__
         out[0] = s_r[0];
         out[1] = s_f[0];
     }
+}
+
+// Plain streaming copy (16-B lanes, grid-stride): the measured-bandwidth
+// yardstick for the pass kernels and the calibration of HBM byte counters.
+__global__ void __launch_bounds__(256) k_copy(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
 }  // namespace ppf

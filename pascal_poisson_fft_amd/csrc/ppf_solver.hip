@@ -54,7 +54,8 @@ struct ppf_plan {
     double mean_scale;
     int zvariant;          // z-pass kernel shape (0 = chosen by n3m; PPF_ZVARIANT overrides)
     // device memory
-    double *W = nullptr;   // spectral workspace (FP64 path)
+    double *W = nullptr;   // spectral workspace (FP64 plans)
+    float *W32 = nullptr;  // spectral workspace (FP32_IR plans)
     double *cs = nullptr;  // Thomas scratch of the one-thread-per-line z kernel
     size_t cs_elems = 0;
     double *zcoef = nullptr;  // z row coefficients of the current solve (2*n3m+4)
@@ -64,7 +65,9 @@ struct ppf_plan {
     double *partial = nullptr;  // residual partial sums
     double *red = nullptr;      // residual totals (2)
     int npartial = 0;
+    double *hred = nullptr;     // pinned host copy of red (refinement convergence test)
     AxisTables<double> tx64, ty64;
+    AxisTables<float> tx32, ty32;
     size_t bytes = 0;
 };
 
@@ -144,36 +147,50 @@ static unsigned persistent_grid(K kernel, int nthreads, int ntiles, size_t dyn_l
     return (unsigned)std::max(1L, std::min<long>(g, ntiles));
 }
 
-template <typename T, int N>
-static void launch_xfwd(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, hipStream_t s) {
+// the workgroup count of a persistent launch that writes per-workgroup partial
+// sums is capped by the partial buffer
+template <typename T, int N, bool PER, bool RES>
+static unsigned launch_xfwd_k(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, const XRes &xr,
+                              hipStream_t s) {
+    using X = XGeom<T, N, PER>;
     const int nlines = p->d.n2m * p->d.n3m;
-    if (p->xkind == AX_R2C) {
-        using X = XGeom<T, N, true>;
-        const unsigned g = persistent_grid(k_xfwd<T, N, true>, X::NT, (nlines + X::LPB - 1) / X::LPB);
-        hipLaunchKernelGGL((k_xfwd<T, N, true>), dim3(g), dim3(X::NT), 0, s, in, W, nlines, p->d.n2m, p->Ly, tb.tw,
-                           tb.wk, tb.qk);
-    } else {
-        using X = XGeom<T, N, false>;
-        const unsigned g = persistent_grid(k_xfwd<T, N, false>, X::NT, (nlines + X::LPB - 1) / X::LPB);
-        hipLaunchKernelGGL((k_xfwd<T, N, false>), dim3(g), dim3(X::NT), 0, s, in, W, nlines, p->d.n2m, p->Ly, tb.tw,
-                           tb.wk, tb.qk);
-    }
+    unsigned g = persistent_grid(k_xfwd<T, N, PER, RES>, X::NT, (nlines + X::LPB - 1) / X::LPB);
+    if (RES) g = std::min<unsigned>(g, (unsigned)p->npartial);
+    hipLaunchKernelGGL((k_xfwd<T, N, PER, RES>), dim3(g), dim3(X::NT), 0, s, in, W, nlines, p->d.n2m, p->Ly, tb.tw,
+                       tb.wk, tb.qk, xr);
+    return g;
+}
+
+template <typename T, int N>
+static unsigned launch_xfwd(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, const XRes *xr,
+                            hipStream_t s) {
+    XRes none{};
+    if (p->xkind == AX_R2C)
+        return xr ? launch_xfwd_k<T, N, true, true>(p, in, W, tb, *xr, s)
+                  : launch_xfwd_k<T, N, true, false>(p, in, W, tb, none, s);
+    return xr ? launch_xfwd_k<T, N, false, true>(p, in, W, tb, *xr, s)
+              : launch_xfwd_k<T, N, false, false>(p, in, W, tb, none, s);
+}
+
+template <typename T, int N, bool PER, bool ACC>
+static void launch_xinv_k(const ppf_plan *p, const T *W, double *P, const AxisTables<T> &tb, const double *dmx1,
+                          hipStream_t s) {
+    using X = XGeom<T, N, PER>;
+    const int nlines = p->Ly * p->Lz;
+    const unsigned g = persistent_grid(k_xinv<T, N, PER, ACC>, X::NT, (nlines + X::LPB - 1) / X::LPB);
+    hipLaunchKernelGGL((k_xinv<T, N, PER, ACC>), dim3(g), dim3(X::NT), 0, s, W, P, nlines, tb.tw, tb.wk, tb.qk, dmx1,
+                       p->mean);
 }
 
 template <typename T, int N>
 static void launch_xinv(const ppf_plan *p, const T *W, double *P, const AxisTables<T> &tb, const double *dmx1,
-                        hipStream_t s) {
-    const int nlines = p->Ly * p->Lz;
+                        bool acc, hipStream_t s) {
     if (p->xkind == AX_R2C) {
-        using X = XGeom<T, N, true>;
-        const unsigned g = persistent_grid(k_xinv<T, N, true>, X::NT, (nlines + X::LPB - 1) / X::LPB);
-        hipLaunchKernelGGL((k_xinv<T, N, true>), dim3(g), dim3(X::NT), 0, s, W, P, nlines, tb.tw, tb.wk, tb.qk, dmx1,
-                           p->mean);
+        if (acc) launch_xinv_k<T, N, true, true>(p, W, P, tb, dmx1, s);
+        else launch_xinv_k<T, N, true, false>(p, W, P, tb, dmx1, s);
     } else {
-        using X = XGeom<T, N, false>;
-        const unsigned g = persistent_grid(k_xinv<T, N, false>, X::NT, (nlines + X::LPB - 1) / X::LPB);
-        hipLaunchKernelGGL((k_xinv<T, N, false>), dim3(g), dim3(X::NT), 0, s, W, P, nlines, tb.tw, tb.wk, tb.qk, dmx1,
-                           p->mean);
+        if (acc) launch_xinv_k<T, N, false, true>(p, W, P, tb, dmx1, s);
+        else launch_xinv_k<T, N, false, false>(p, W, P, tb, dmx1, s);
     }
 }
 
@@ -202,10 +219,15 @@ static void launch_y(const ppf_plan *p, T *W, const AxisTables<T> &tb, bool inve
 }
 
 template <typename T>
-static int dispatch_xfwd(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, hipStream_t s) {
+static int dispatch_xfwd(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, const XRes *xr,
+                         hipStream_t s, unsigned *grid = nullptr) {
+    unsigned g = 0;
     switch (p->d.n1m) {
 #define X_CASE(n) \
-    case n: launch_xfwd<T, n>(p, in, W, tb, s); return PPF_OK;
+    case n:                                            \
+        g = launch_xfwd<T, n>(p, in, W, tb, xr, s);   \
+        if (grid) *grid = g;                           \
+        return PPF_OK;
         PPF_FOR_SIZES(X_CASE)
 #undef X_CASE
     }
@@ -213,10 +235,10 @@ static int dispatch_xfwd(const ppf_plan *p, const double *in, T *W, const AxisTa
 }
 template <typename T>
 static int dispatch_xinv(const ppf_plan *p, const T *W, double *P, const AxisTables<T> &tb, const double *dmx1,
-                         hipStream_t s) {
+                         bool acc, hipStream_t s) {
     switch (p->d.n1m) {
 #define X_CASE(n) \
-    case n: launch_xinv<T, n>(p, W, P, tb, dmx1, s); return PPF_OK;
+    case n: launch_xinv<T, n>(p, W, P, tb, dmx1, acc, s); return PPF_OK;
         PPF_FOR_SIZES(X_CASE)
 #undef X_CASE
     }
@@ -234,7 +256,8 @@ static int dispatch_y(const ppf_plan *p, T *W, const AxisTables<T> &tb, bool inv
     return PPF_ERR_SIZE;
 }
 
-static int launch_z(ppf_plan *p, double *W, const double *dx3, const double *dmx3, hipStream_t s) {
+template <typename T>
+static int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hipStream_t s) {
     ZArgs z;
     z.Lx = p->Lx;
     z.Ly = p->Ly;
@@ -249,6 +272,8 @@ static int launch_z(ppf_plan *p, double *W, const double *dx3, const double *dmx
     z.mean_jp = p->mean_jp;
     z.mean_scale = p->mean_scale;
     const int n3m = p->d.n3m;
+    // z row coefficients of this solve (also read by the refinement residual)
+    hipLaunchKernelGGL(k_zcoef, dim3((n3m + 255) / 256), dim3(256), 0, s, dx3, dmx3, n3m, p->zcoef, p->zcoef + n3m);
     // partitioned register-resident Thomas, shape (K rows per thread, S
     // columns per workgroup); launched only when S*P <= NTMAX and the LDS
     // footprint stays within 64 KiB, else the one-thread-per-line kernel
@@ -260,9 +285,8 @@ static int launch_z(ppf_plan *p, double *W, const double *dx3, const double *dmx
         const size_t shm = sizeof(double) * (2 * (size_t)P * K + 8 * (size_t)P * S);
         if (P < 1 || S * P > NTMAX || shm > 65536) return;
         const int tiles = (p->Lx + S - 1) / S;
-        double *zam = p->zcoef, *zap = p->zcoef + n3m;
-        hipLaunchKernelGGL(k_zcoef, dim3((n3m + 255) / 256), dim3(256), 0, s, dx3, dmx3, n3m, zam, zap);
-        hipLaunchKernelGGL((k_zpart<double, K, S, NTMAX, WPE>), dim3((unsigned)(tiles * p->znjr)), dim3(S * P), shm,
+        const double *zam = p->zcoef, *zap = p->zcoef + n3m;
+        hipLaunchKernelGGL((k_zpart<T, K, S, NTMAX, WPE>), dim3((unsigned)(tiles * p->znjr)), dim3(S * P), shm,
                            s, W, z, zam, zap, dmx3, p->mean, P);
         done = true;
     };
@@ -272,16 +296,16 @@ static int launch_z(ppf_plan *p, double *W, const double *dx3, const double *dmx
         const int P = (n3m + K - 1) / K;
         const size_t shm = sizeof(double) * (2 * (size_t)P * K + 3 * (size_t)S * 65 + (size_t)P * S);
         if (P < 1 || P > 32 || K < 2 || shm > 65536) return;
-        double *zam = p->zcoef, *zap = p->zcoef + n3m;
-        hipLaunchKernelGGL(k_zcoef, dim3((n3m + 255) / 256), dim3(256), 0, s, dx3, dmx3, n3m, zam, zap);
+        const double *zam = p->zcoef, *zap = p->zcoef + n3m;
         const int ntiles = ((p->Lx + S - 1) / S) * p->znjr;
-        const unsigned g = persistent_grid(k_zpipe<double, K, S, WPE>, S * P, ntiles, shm);
-        hipLaunchKernelGGL((k_zpipe<double, K, S, WPE>), dim3(g), dim3(S * P), shm, s, W, z, zam, zap, p->mean, P);
+        const unsigned g = persistent_grid(k_zpipe<T, K, S, WPE>, S * P, ntiles, shm);
+        hipLaunchKernelGGL((k_zpipe<T, K, S, WPE>), dim3(g), dim3(S * P), shm, s, W, z, zam, zap, p->mean, P);
         done = true;
     };
     using std::integral_constant;
     int v = p->zvariant;
-    if (v == 0) v = (n3m <= 512) ? 7 : (n3m <= 1024 ? 3 : 4);
+    // measured on MI355X at 512^3: (K=32, S=16) 0.54 ms, pipelined K=16 0.81 ms
+    if (v == 0) v = (n3m <= 1024) ? 3 : 4;
     switch (v) {
         case 7: pipe(integral_constant<int, 16>{}, integral_constant<int, 16>{}, integral_constant<int, 2>{}); break;
         case 8: pipe(integral_constant<int, 16>{}, integral_constant<int, 8>{}, integral_constant<int, 2>{}); break;
@@ -304,7 +328,8 @@ static int launch_z(ppf_plan *p, double *W, const double *dx3, const double *dmx
         if (p->cs_elems < (size_t)ncol * (size_t)n3m) return PPF_ERR_UNSUPPORTED;
         const int nt = 256;
         const size_t shm = 2 * sizeof(double) * (size_t)n3m;
-        hipLaunchKernelGGL((k_zthomas<double>), dim3((unsigned)((ncol + nt - 1) / nt)), dim3(nt), shm, s, W, p->cs, z,
+        hipLaunchKernelGGL((k_zthomas<T>), dim3((unsigned)((ncol + nt - 1) / nt)), dim3(nt), shm, s, W,
+                           reinterpret_cast<T *>(p->cs), z,
                            dx3, dmx3, p->mean);
     }
     return PPF_OK;
@@ -312,10 +337,13 @@ static int launch_z(ppf_plan *p, double *W, const double *dx3, const double *dmx
 
 static void free_plan(ppf_plan *p) {
     if (!p) return;
-    void *ptrs[] = {p->W, p->cs, p->zcoef, p->lamx, p->lamy, p->zx, p->zy, p->mean, p->partial, p->red,
-                    p->tx64.tw, p->tx64.wk, p->tx64.qk, p->ty64.tw, p->ty64.wk, p->ty64.qk};
+    void *ptrs[] = {p->W,        p->W32,      p->cs,       p->zcoef,    p->lamx,     p->lamy,     p->zx,
+                    p->zy,       p->mean,     p->partial,  p->red,      p->tx64.tw,  p->tx64.wk,  p->tx64.qk,
+                    p->ty64.tw,  p->ty64.wk,  p->ty64.qk,  p->tx32.tw,  p->tx32.wk,  p->tx32.qk,  p->ty32.tw,
+                    p->ty32.wk,  p->ty32.qk};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
+    if (p->hred) (void)hipHostFree(p->hred);
     delete p;
 }
 
@@ -348,7 +376,6 @@ int ppf_plan_create(const ppf_desc *desc, ppf_plan **out) {
     if (!size_supported(d.n1m) || !size_supported(d.n2m) || d.n3m < 2 || (d.n3m & 1) || d.n3m > 4096)
         return PPF_ERR_SIZE;
     if (d.pbc3) return PPF_ERR_UNSUPPORTED;  // cyclic z Thomas not provided
-    if (d.precision != PPF_PREC_FP64) return PPF_ERR_UNSUPPORTED;
 
     ppf_plan *p = new (std::nothrow) ppf_plan();
     if (!p) return PPF_ERR_ALLOC;
@@ -371,21 +398,29 @@ int ppf_plan_create(const ppf_desc *desc, ppf_plan **out) {
     p->mean_scale = 1.0 / (fx * fy * (double)d.n1m * (double)d.n2m * (double)d.n3m);
 
     int rc = PPF_OK;
+    const bool fp32 = d.precision == PPF_PREC_FP32_IR;
     const size_t wsz = (size_t)p->Lx * p->Ly * p->Lz;
     // Thomas scratch only where the one-thread-per-line kernel can run: long
     // lines (n3m > 2048) or when it is selected explicitly
     const bool need_cs = d.n3m > 2048 || p->zvariant == 6;
     const size_t csz = need_cs ? (size_t)p->Lx * p->znjr * d.n3m : 1;
     p->cs_elems = csz;
-    if ((rc = dev_alloc(p, &p->W, wsz)) || (rc = dev_alloc(p, &p->cs, csz)) || (rc = dev_alloc(p, &p->lamx, p->Lx)) ||
+    if ((rc = fp32 ? dev_alloc(p, &p->W32, wsz) : dev_alloc(p, &p->W, wsz)) || (rc = dev_alloc(p, &p->cs, csz)) || (rc = dev_alloc(p, &p->lamx, p->Lx)) ||
         (rc = dev_alloc(p, &p->lamy, p->Ly)) || (rc = dev_alloc(p, &p->zx, p->Lx)) || (rc = dev_alloc(p, &p->zy, p->Ly)) ||
         (rc = dev_alloc(p, &p->mean, 1)) || (rc = dev_alloc(p, &p->red, 2)) ||
         (rc = dev_alloc(p, &p->zcoef, 2 * (size_t)d.n3m + 4))) {
         free_plan(p);
         return rc;
     }
-    p->npartial = 1024;
+    // partial sums: one pair per workgroup of the residual kernels (persistent
+    // grids are capped at this count)
+    p->npartial = 4096;
     if ((rc = dev_alloc(p, &p->partial, 2 * (size_t)p->npartial))) { free_plan(p); return rc; }
+    if (hipHostMalloc((void **)&p->hred, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        p->hred = nullptr;
+        free_plan(p);
+        return PPF_ERR_ALLOC;
+    }
 
     // eigenvalue and zero-mode tables per scalar column / row
     const double h1 = d.L1 / d.n1m, h2 = d.L2 / d.n2m;
@@ -414,7 +449,9 @@ int ppf_plan_create(const ppf_desc *desc, ppf_plan **out) {
         free_plan(p);
         return PPF_ERR_HIP;
     }
-    if ((rc = upload_tables(p, p->tx64, d.n1m, p->xkind)) || (rc = upload_tables(p, p->ty64, d.n2m, p->ykind))) {
+    if (fp32) rc = upload_tables(p, p->tx32, d.n1m, p->xkind) ? PPF_ERR_ALLOC : upload_tables(p, p->ty32, d.n2m, p->ykind);
+    else rc = upload_tables(p, p->tx64, d.n1m, p->xkind) ? PPF_ERR_ALLOC : upload_tables(p, p->ty64, d.n2m, p->ykind);
+    if (rc) {
         free_plan(p);
         return rc;
     }
@@ -430,17 +467,109 @@ int ppf_plan_destroy(ppf_plan *plan) {
 
 size_t ppf_plan_workspace_bytes(const ppf_plan *plan) { return plan ? plan->bytes : 0; }
 
+}  // extern "C"
+
+// The five passes of one solve of the plan's precision.  xr != nullptr: pass 1
+// transforms the residual f - L P (refinement form); acc: pass 5 adds to P.
+// ev (7 events or nullptr) brackets the passes for ppf_solve_timed.
+template <typename T>
+static int run_passes(ppf_plan *p, T *W, const AxisTables<T> &tx, const AxisTables<T> &ty, const double *in,
+                      const XRes *xr, double *P, bool acc, const double *dmx1, const double *dmx2, const double *dmx3,
+                      const double *dx3, hipStream_t s, hipEvent_t *ev, unsigned *res_grid) {
+    int rc;
+    if (ev) PPF_HIP(hipEventRecord(ev[0], s));
+    if ((rc = dispatch_xfwd<T>(p, in, W, tx, xr, s, res_grid))) return rc;
+    if (ev) PPF_HIP(hipEventRecord(ev[1], s));
+    if ((rc = dispatch_y<T>(p, W, ty, false, dmx2, s))) return rc;
+    if (ev) PPF_HIP(hipEventRecord(ev[2], s));
+    if ((rc = launch_z<T>(p, W, dx3, dmx3, s))) return rc;
+    if (ev) PPF_HIP(hipEventRecord(ev[3], s));
+    if ((rc = dispatch_y<T>(p, W, ty, true, dmx2, s))) return rc;
+    if (ev) PPF_HIP(hipEventRecord(ev[4], s));
+    if ((rc = dispatch_xinv<T>(p, W, P, tx, dmx1, acc, s))) return rc;
+    if (ev) PPF_HIP(hipEventRecord(ev[5], s));
+    PPF_HIP(hipGetLastError());
+    return PPF_OK;
+}
+
+static XRes make_xres(const ppf_plan *p, const double *P) {
+    XRes r{};
+    r.P = P;
+    r.zam = p->zcoef;
+    r.zap = p->zcoef + p->d.n3m;
+    r.partial = p->partial;
+    const double h1 = p->d.L1 / p->d.n1m, h2 = p->d.L2 / p->d.n2m;
+    r.ih1 = 1.0 / (h1 * h1);
+    r.ih2 = 1.0 / (h2 * h2);
+    r.pbc1 = p->d.pbc1 != 0;
+    r.pbc2 = p->d.pbc2 != 0;
+    return r;
+}
+
+// FP32 transforms and Thomas storage with FP64 refinement: x_0 = solve32(f);
+// then r = f - L x (FP64, fused into pass 1), x += solve32(r) (fused into
+// pass 5) until ||r||/||f|| <= tol or max_sweeps corrections.  The z pass
+// computes in FP64 on FP32 data, so the pinned (0,0) system and the mean stay
+// FP64.  tol < 0: exactly max_sweeps corrections, no convergence test.
+static int solve_fp32_ir(ppf_plan *p, const double *f, double *P, const double *dmx1, const double *dmx2,
+                         const double *dmx3, const double *dx3, hipStream_t s, ppf_stats *stats) {
+    const double tol = p->d.tol == 0.0 ? 1e-10 : p->d.tol;
+    const int maxs = p->d.max_sweeps > 0 ? p->d.max_sweeps : 8;
+    int rc;
+    if ((rc = run_passes<float>(p, p->W32, p->tx32, p->ty32, f, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, nullptr,
+                                nullptr)))
+        return rc;
+    const XRes xr = make_xres(p, P);
+    int sweeps = 0;
+    double rel = -1.0;
+    bool converged = false;
+    if (tol < 0) {
+        for (; sweeps < maxs; ++sweeps)
+            if ((rc = run_passes<float>(p, p->W32, p->tx32, p->ty32, f, &xr, P, true, dmx1, dmx2, dmx3, dx3, s,
+                                        nullptr, nullptr)))
+                return rc;
+        if (stats) {
+            if ((rc = ppf_residual(p, f, P, dmx3, dx3, s, &rel))) return rc;
+        }
+        converged = true;
+    } else {
+        for (;;) {
+            // residual of the current iterate, transformed by pass 1
+            unsigned g = 0;
+            if ((rc = dispatch_xfwd<float>(p, f, p->W32, p->tx32, &xr, s, &g))) return rc;
+            hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(256), 0, s, p->partial, (int)g, p->red);
+            PPF_HIP(hipMemcpyAsync(p->hred, p->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+            PPF_HIP(hipStreamSynchronize(s));
+            rel = p->hred[1] > 0 ? std::sqrt(p->hred[0] / p->hred[1]) : std::sqrt(p->hred[0]);
+            if (rel <= tol) { converged = true; break; }
+            if (sweeps == maxs || !(rel == rel)) break;
+            if ((rc = dispatch_y<float>(p, p->W32, p->ty32, false, dmx2, s)) ||
+                (rc = launch_z<float>(p, p->W32, dx3, dmx3, s)) ||
+                (rc = dispatch_y<float>(p, p->W32, p->ty32, true, dmx2, s)) ||
+                (rc = dispatch_xinv<float>(p, p->W32, P, p->tx32, dmx1, true, s)))
+                return rc;
+            PPF_HIP(hipGetLastError());
+            ++sweeps;
+        }
+    }
+    if (stats) {
+        stats->sweeps = sweeps;
+        stats->rel_residual = rel;
+    }
+    return converged ? PPF_OK : PPF_ERR_NOT_CONVERGED;
+}
+
+extern "C" {
+
 int ppf_solve(ppf_plan *p, const double *prhs, double *P, const double *dmx1, const double *dmx2, const double *dmx3,
               const double *dx3, void *stream, ppf_stats *stats) {
     if (!p || !prhs || !P || !dmx1 || !dmx2 || !dmx3 || !dx3) return PPF_ERR_INVALID_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (p->d.precision == PPF_PREC_FP32_IR) return solve_fp32_ir(p, prhs, P, dmx1, dmx2, dmx3, dx3, s, stats);
     int rc;
-    if ((rc = dispatch_xfwd<double>(p, prhs, p->W, p->tx64, s))) return rc;
-    if ((rc = dispatch_y<double>(p, p->W, p->ty64, false, dmx2, s))) return rc;
-    if ((rc = launch_z(p, p->W, dx3, dmx3, s))) return rc;
-    if ((rc = dispatch_y<double>(p, p->W, p->ty64, true, dmx2, s))) return rc;
-    if ((rc = dispatch_xinv<double>(p, p->W, P, p->tx64, dmx1, s))) return rc;
-    PPF_HIP(hipGetLastError());
+    if ((rc = run_passes<double>(p, p->W, p->tx64, p->ty64, prhs, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s,
+                                 nullptr, nullptr)))
+        return rc;
     if (stats) { stats->sweeps = 0; stats->rel_residual = -1.0; }
     return PPF_OK;
 }
@@ -450,27 +579,25 @@ int ppf_solve_timed(ppf_plan *p, const double *prhs, double *P, const double *dm
     if (!p || !prhs || !P || !dmx1 || !dmx2 || !dmx3 || !dx3 || !stage_ms) return PPF_ERR_INVALID_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipEvent_t ev[6];
-    for (int i = 0; i < 6; ++i) PPF_HIP(hipEventCreate(&ev[i]));
+    int made = 0;
     int rc = PPF_OK;
-    (void)hipEventRecord(ev[0], s);
-    if (!rc) rc = dispatch_xfwd<double>(p, prhs, p->W, p->tx64, s);
-    (void)hipEventRecord(ev[1], s);
-    if (!rc) rc = dispatch_y<double>(p, p->W, p->ty64, false, dmx2, s);
-    (void)hipEventRecord(ev[2], s);
-    if (!rc) rc = launch_z(p, p->W, dx3, dmx3, s);
-    (void)hipEventRecord(ev[3], s);
-    if (!rc) rc = dispatch_y<double>(p, p->W, p->ty64, true, dmx2, s);
-    (void)hipEventRecord(ev[4], s);
-    if (!rc) rc = dispatch_xinv<double>(p, p->W, P, p->tx64, dmx1, s);
-    (void)hipEventRecord(ev[5], s);
-    if (hipEventSynchronize(ev[5]) != hipSuccess && !rc) rc = PPF_ERR_HIP;
-    for (int i = 0; i < 5; ++i) {
+    for (; made < 6; ++made)
+        if (hipEventCreate(&ev[made]) != hipSuccess) { rc = PPF_ERR_HIP; break; }
+    if (!rc) {
+        if (p->d.precision == PPF_PREC_FP32_IR)
+            rc = run_passes<float>(p, p->W32, p->tx32, p->ty32, prhs, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, ev,
+                                   nullptr);
+        else
+            rc = run_passes<double>(p, p->W, p->tx64, p->ty64, prhs, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, ev,
+                                    nullptr);
+    }
+    if (!rc && hipEventSynchronize(ev[5]) != hipSuccess) rc = PPF_ERR_HIP;
+    for (int i = 0; i < 5 && !rc; ++i) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess && !rc) rc = PPF_ERR_HIP;
+        if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) rc = PPF_ERR_HIP;
         stage_ms[i] = ms;
     }
-    for (int i = 0; i < 6; ++i) (void)hipEventDestroy(ev[i]);
-    if (!rc && hipGetLastError() != hipSuccess) rc = PPF_ERR_HIP;
+    for (int i = 0; i < made; ++i) (void)hipEventDestroy(ev[i]);
     return rc;
 }
 
@@ -478,18 +605,16 @@ int ppf_residual(ppf_plan *p, const double *f, const double *P, const double *dm
                  double *rel) {
     if (!p || !f || !P || !dmx3 || !dx3 || !rel) return PPF_ERR_INVALID_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    ResArgs a;
-    a.n1m = p->d.n1m; a.n2m = p->d.n2m; a.n3m = p->d.n3m;
-    a.pbc1 = p->d.pbc1 != 0; a.pbc2 = p->d.pbc2 != 0;
-    const double h1 = p->d.L1 / p->d.n1m, h2 = p->d.L2 / p->d.n2m;
-    a.ih1 = 1.0 / (h1 * h1);
-    a.ih2 = 1.0 / (h2 * h2);
-    hipLaunchKernelGGL(k_residual, dim3(p->npartial), dim3(256), 0, s, f, P, a, dx3, dmx3, (double *)nullptr, p->partial);
-    hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(256), 0, s, p->partial, p->npartial, p->red);
-    double h[2];
-    PPF_HIP(hipMemcpyAsync(h, p->red, sizeof(h), hipMemcpyDeviceToHost, s));
+    const int n3m = p->d.n3m;
+    hipLaunchKernelGGL(k_zcoef, dim3((n3m + 255) / 256), dim3(256), 0, s, dx3, dmx3, n3m, p->zcoef, p->zcoef + n3m);
+    const XRes xr = make_xres(p, P);
+    const int g = std::min(p->npartial, p->d.n2m * n3m);
+    hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, s, f, xr, p->d.n1m, p->d.n2m, n3m);
+    hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(256), 0, s, p->partial, g, p->red);
+    PPF_HIP(hipGetLastError());
+    PPF_HIP(hipMemcpyAsync(p->hred, p->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     PPF_HIP(hipStreamSynchronize(s));
-    *rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+    *rel = p->hred[1] > 0 ? std::sqrt(p->hred[0] / p->hred[1]) : std::sqrt(p->hred[0]);
     return PPF_OK;
 }
 
@@ -518,5 +643,29 @@ int ppf_device_count(int *count) {
     return hipGetDeviceCount(count) == hipSuccess ? PPF_OK : PPF_ERR_HIP;
 }
 int ppf_set_device(int device) { return hipSetDevice(device) == hipSuccess ? PPF_OK : PPF_ERR_HIP; }
+
+int ppf_copy_timed(double *dst, const double *src, size_t count, int reps, void *stream, float *ms) {
+    if (!dst || !src || reps <= 0 || !ms || (count & 1)) return PPF_ERR_INVALID_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t n2 = count / 2;
+    const unsigned grid = 8u * (unsigned)num_cus();
+    hipEvent_t e0, e1;
+    PPF_HIP(hipEventCreate(&e0));
+    PPF_HIP(hipEventCreate(&e1));
+    int rc = PPF_OK;
+    if (hipEventRecord(e0, s) != hipSuccess) rc = PPF_ERR_HIP;
+    for (int r = 0; r < reps && rc == PPF_OK; ++r) {
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, reinterpret_cast<double2 *>(dst),
+                           reinterpret_cast<const double2 *>(src), n2);
+        if (hipGetLastError() != hipSuccess) rc = PPF_ERR_HIP;
+    }
+    if (rc == PPF_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+        rc = PPF_ERR_HIP;
+    if (rc == PPF_OK) *ms /= (float)reps;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
 
 }  // extern "C"
