@@ -99,6 +99,59 @@ int ppf_solve_timed(ppf_plan *plan, const double *prhs, double *p, const double 
  * (fft_3d_poisson.f90:123-130). */
 int ppf_plan_destroy(ppf_plan *plan);
 
+/* ppf_solve with exactly `sweeps` refinement corrections and no convergence
+ * test (FP32_IR plans; FP64 plans need sweeps = 0).  Only kernel launches on
+ * `stream`: safe inside hipStreamBeginCapture.  No counterpart in the
+ * reference (its solve is FP64 only); used by the batched solve's graph. */
+int ppf_solve_sweeps(ppf_plan *plan, const double *prhs, double *p, const double *dmx1, const double *dmx2,
+                     const double *dmx3, const double *dx3, int sweeps, void *stream);
+
+/* Autotuner: times the z-pass kernel shapes (partitioned Thomas with K rows
+ * per thread and S columns per workgroup, and the pipelined wavefront cyclic
+ * reduction) on this plan's extents with full solves of the given data, keeps
+ * the fastest for later solves and returns its id in *zvariant (may be NULL).
+ * p is overwritten.  Synchronises the stream.  Replaces the reference's fixed
+ * thread shapes threads_tdma / threads_fft (fft_3d_poisson.f90:70-89), which
+ * the caller had to choose by hand. */
+int ppf_plan_tune(ppf_plan *plan, const double *prhs, double *p, const double *dmx1, const double *dmx2,
+                  const double *dmx3, const double *dx3, void *stream, int *zvariant);
+
+/* Pins the z-pass kernel shape (an id returned by ppf_plan_tune; 0 = the
+ * default chosen by n3m). */
+int ppf_plan_set_zvariant(ppf_plan *plan, int zvariant);
+
+/* ---- batched solves of independent problems ----------------------------- */
+/* Device buffers of one problem of a batch (as in ppf_solve). */
+typedef struct ppf_batch_io {
+    const double *prhs;
+    double *p;
+    const double *dmx1, *dmx2, *dmx3, *dx3;
+} ppf_batch_io;
+
+#define PPF_BATCH_NO_GRAPH 1 /* launch every call directly (no hipGraph) */
+#define PPF_BATCH_NO_TUNE 2  /* skip the per-shape z-pass autotuner */
+
+typedef struct ppf_batch ppf_batch;
+
+/* Many independent problems of any extents and boundary conditions, bound to
+ * their device buffers.  One plan per problem; problems with equal extents
+ * and BCs share one autotuner result.  The reference solves one problem per
+ * plan and per call (module singleton, fft_3d_poisson.f90:43-55); this has no
+ * counterpart there.
+ * The first ppf_batch_solve solves every problem directly (FP32_IR problems
+ * with their convergence test), records the sweeps each needed, and captures
+ * the whole batch -- every problem as its own branch, FP32_IR problems with
+ * those sweep counts -- into one hipGraph; later calls replay the graph. */
+int ppf_batch_create(int nprob, const ppf_desc *descs, const ppf_batch_io *io, int flags, ppf_batch **batch);
+
+/* Solves every problem of the batch on `stream`.  stats (nprob entries or
+ * NULL) receives per-problem sweeps and, when given, the measured relative
+ * residual (which synchronises).  Without stats a graph replay returns
+ * without synchronising. */
+int ppf_batch_solve(ppf_batch *batch, void *stream, ppf_stats *stats);
+
+int ppf_batch_destroy(ppf_batch *batch);
+
 /* Relative FP64 residual ||f - L p||_2 / ||f||_2 of the 7-point operator the
  * solve inverts (x/y: uniform spacing with zero-flux walls or wrap, the
  * operators whose eigenvalues are fft_3d_poisson.f90:288-351; z: the stretched
@@ -124,6 +177,12 @@ typedef struct ppf_tdma_plan ppf_tdma_plan;
 int ppf_tdma_plan_many_create(int nx_sys, int ny_sys, int nz_row, ppf_tdma_plan **plan);
 int ppf_tdma_many_solve(ppf_tdma_plan *plan, const double *a, const double *b, double *c, double *d, void *stream);
 int ppf_tdma_plan_many_destroy(ppf_tdma_plan *plan);
+/* Cyclic systems (row 0 couples to x_{n-1} through a, row n-1 to x_0 through
+ * c).  Replaces PaScaL_TDMA_many_solve_cycle_cuda for nprocs = 1
+ * (pascal_tdma_cuda.f90:279-318) -> tdma_cycl_many_cuda (tdmas_cuda.f90:
+ * 114-217), same operation order; needs nz_row >= 2. */
+int ppf_tdma_many_solve_cycle(ppf_tdma_plan *plan, const double *a, const double *b, double *c, double *d,
+                              void *stream);
 
 /* ---- device memory shim for hosts without HIP bindings (Fortran) -------- */
 /* Stand-ins for the CUDA Fortran `device` arrays, allocate/deallocate and

@@ -44,6 +44,7 @@ def _load() -> C.CDLL:
     lib.or_dct2.argtypes = [I, _dp, _dp]
     lib.or_idct2.argtypes = [I, _dp, _dp]
     lib.or_tdma_many.argtypes = [_dp, _dp, _dp, _dp, C.c_long, I]
+    lib.or_tdma_cycl_many.argtypes = [_dp, _dp, _dp, _dp, C.c_long, I]
     lib.or_solve.argtypes = [I, I, I, D, D, D, I, I, I, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.or_solve.restype = I
     lib.or_apply_L.argtypes = [I, I, I, D, D, I, I, _dp, _dp, _dp, _dp]
@@ -134,8 +135,33 @@ def tdma_many(a, b, c, d):
     return d
 
 
+def tdma_cycl_many(a, b, c, d):
+    """Cyclic Thomas on arrays of shape (nrow, nsys); returns the solution."""
+    a, b = _f64(a), _f64(b)
+    c, d = _f64(c).copy(), _f64(d).copy()
+    nrow, nsys = d.shape
+    lib().or_tdma_cycl_many(a, b, c, d, nsys, nrow)
+    return d
+
+
 def ref_tdma_available() -> bool:
     return os.path.exists(_REF_TDMAS)
+
+
+def ref_tdma_cycl_many(a, b, c, d):
+    """The reference's own tdma_cycl_many (00_C/PaScaL_TDMA/src/tdmas.cpp:161-210),
+    compiled from source into oracle/_ref.  It strides rows by n2 (the row
+    count) where tdma_many strides by n1, so it agrees with the Fortran
+    tdma_cycl_many layout only for square batches (nsys == nrow)."""
+    rl = C.CDLL(_REF_TDMAS)
+    rl.tdma_cycl_many.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int]
+    a, b = _f64(a).copy(), _f64(b).copy()
+    c, d = _f64(c).copy(), _f64(d).copy()
+    nrow, nsys = d.shape
+    if nrow != nsys:
+        raise ValueError("the reference tdma_cycl_many is layout-consistent only for nsys == nrow")
+    rl.tdma_cycl_many(a, b, c, d, nsys, nrow)
+    return d
 
 
 def ref_tdma_many(a, b, c, d):

@@ -273,6 +273,53 @@ void or_tdma_many(const double *a, const double *b, double *c, double *d, long n
 }
 
 /* ------------------------------------------------------------------------- */
+/* Cyclic Thomas for many systems (periodic rows: a_0 couples to x_{n-1},       */
+/* c_{n-1} to x_0), op order of tdma_cycl_many, 01_Fortran/PaScaL_TDMA/src/     */
+/* tdmas.f90:160-213 (= tdmas_cuda.f90:114-217): rows 1..n-1 are solved for     */
+/* d' and the correction vector e (e_1 = -a_1, e_{n-1} = -c_{n-1}), then x_0,   */
+/* then x_k = d'_k + x_0 e_k.  Same layout as or_tdma_many; c, d overwritten.   */
+/* Needs nrow >= 2.                                                            */
+/* ------------------------------------------------------------------------- */
+void or_tdma_cycl_many(const double *a, const double *b, double *c, double *d, long nsys, int nrow) {
+    const size_t ns = (size_t)nsys;
+#pragma omp parallel
+    {
+        double *e = (double *)malloc(sizeof(double) * (size_t)nrow);
+#pragma omp for schedule(static)
+        for (long s = 0; s < nsys; ++s) {
+            for (int k = 0; k < nrow; ++k) e[k] = 0.0;
+            e[1] = -a[ns + (size_t)s];
+            e[nrow - 1] = -c[(size_t)(nrow - 1) * ns + (size_t)s];
+            {
+                const size_t q = ns + (size_t)s;
+                d[q] = d[q] / b[q];
+                e[1] = e[1] / b[q];
+                c[q] = c[q] / b[q];
+            }
+            for (int k = 2; k < nrow; ++k) {
+                const size_t q = (size_t)k * ns + (size_t)s, qm = q - ns;
+                const double rr = 1.0 / (b[q] - a[q] * c[qm]);
+                d[q] = rr * (d[q] - a[q] * d[qm]);
+                e[k] = rr * (e[k] - a[q] * e[k - 1]);
+                c[q] = rr * c[q];
+            }
+            for (int k = nrow - 2; k >= 1; --k) {
+                const size_t q = (size_t)k * ns + (size_t)s;
+                d[q] = d[q] - c[q] * d[q + ns];
+                e[k] = e[k] - c[q] * e[k + 1];
+            }
+            const size_t q0 = (size_t)s, q1 = ns + (size_t)s, qn = (size_t)(nrow - 1) * ns + (size_t)s;
+            d[q0] = (d[q0] - a[q0] * d[qn] - c[q0] * d[q1]) / (b[q0] + a[q0] * e[nrow - 1] + c[q0] * e[1]);
+            for (int k = 1; k < nrow; ++k) {
+                const size_t q = (size_t)k * ns + (size_t)s;
+                d[q] = d[q] + d[q0] * e[k];
+            }
+        }
+        free(e);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
 /* Line-transform drivers over a 3-D field.                                    */
 /* ------------------------------------------------------------------------- */
 typedef struct { double *v; cplx *spec, *work, *tmp, *tw; double *lin, *lout; cplx *cin; } scratch_t;
@@ -462,13 +509,17 @@ static void finish_solution(const double *sol, double *P, int n1m, int n2m, int 
 /* ------------------------------------------------------------------------- */
 /* The solve: cuda_Poisson_FFT_1D (fft_3d_poisson.f90:353-626), one rank.       */
 /* rhs: (n1m,n2m,n3m) x-fastest, not modified.  P: (n1m+2,n2m+2,n3m+2).         */
-/* Returns 0, or a negative code for unsupported input (odd n, periodic z).    */
+/* Returns 0, or a negative code for unsupported input (odd n).                */
+/* Periodic z (pbc3) follows the reference as written: the z rows are built   */
+/* exactly as for walls (am = 0 at k = 1 and ap = 0 at k = n3m on the only     */
+/* rank, :1143-1146, whatever pbc3 says) and solved by the cyclic Thomas        */
+/* (cuda_ptdma_core, :898-913 -> tdma_cycl_many_cuda), so the corner            */
+/* couplings are zero; the z ghost planes are wrapped instead of extrapolated. */
 /* ------------------------------------------------------------------------- */
 int or_solve(int n1m, int n2m, int n3m, double L1, double L2, double L3, int pbc1, int pbc2, int pbc3,
              const double *rhs, double *P, const double *dmx1, const double *dmx2, const double *dmx3, const double *dx3) {
     (void)L3;
     if (n1m < 2 || n2m < 2 || n3m < 2 || (n1m & 1) || (n2m & 1) || (n3m & 1)) return -2;
-    if (pbc3) return -5; /* cyclic z (PaScaL_TDMA_many_solve_cycle_cuda) is not restated here */
     const size_t N = (size_t)n1m * n2m * n3m;
     double *dxk2 = (double *)malloc(sizeof(double) * (size_t)n1m);
     double *dyk2 = (double *)malloc(sizeof(double) * (size_t)n2m);
@@ -496,7 +547,8 @@ int or_solve(int n1m, int n2m, int n3m, double L1, double L2, double L3, int pbc
                     }
             }
             a[0] = 0.0; b[0] = 1.0; c[0] = 0.0; sol[0] = 0.0; /* pin (:1152-1157) */
-            or_tdma_many(a, b, c, sol, nsys, n3m);
+            if (pbc3) or_tdma_cycl_many(a, b, c, sol, nsys, n3m);
+            else or_tdma_many(a, b, c, sol, nsys, n3m);
             free(a); free(b); free(c);
         }
         real_transform(1, 1, sol, n1m, n2m, n3m);
@@ -541,7 +593,8 @@ int or_solve(int n1m, int n2m, int n3m, double L1, double L2, double L3, int pbc
                     }
             }
             a[0] = 0.0; b[0] = 1.0; c[0] = 0.0; d[0] = 0.0;
-            or_tdma_many(a, b, c, d, nsys, n3m);
+            if (pbc3) or_tdma_cycl_many(a, b, c, d, nsys, n3m);
+            else or_tdma_many(a, b, c, d, nsys, n3m);
         }
         for (size_t q = 0; q < NC; ++q) S[q] = dr[q] + I * di[q];
         free(a); free(b); free(c); free(dr); free(di);

@@ -50,6 +50,15 @@ class Stats(C.Structure):
     _fields_ = [("sweeps", C.c_int), ("rel_residual", C.c_double)]
 
 
+class BatchIO(C.Structure):
+    _fields_ = [("prhs", C.c_void_p), ("p", C.c_void_p), ("dmx1", C.c_void_p), ("dmx2", C.c_void_p),
+                ("dmx3", C.c_void_p), ("dx3", C.c_void_p)]
+
+
+PPF_BATCH_NO_GRAPH = 1
+PPF_BATCH_NO_TUNE = 2
+
+
 _lib = None
 
 
@@ -74,6 +83,7 @@ def lib() -> C.CDLL:
     L.ppf_tdma_plan_many_create.argtypes = [I, I, I, C.POINTER(vp)]
     L.ppf_tdma_many_solve.argtypes = [vp, vp, vp, vp, vp, vp]
     L.ppf_tdma_plan_many_destroy.argtypes = [vp]
+    L.ppf_tdma_many_solve_cycle.argtypes = [vp, vp, vp, vp, vp, vp]
     L.ppf_malloc.argtypes = [C.POINTER(vp), sz]
     L.ppf_free.argtypes = [vp]
     L.ppf_memcpy.argtypes = [vp, vp, sz, I]
@@ -82,6 +92,12 @@ def lib() -> C.CDLL:
     L.ppf_device_count.argtypes = [C.POINTER(I)]
     L.ppf_set_device.argtypes = [I]
     L.ppf_copy_timed.argtypes = [vp, vp, sz, I, vp, C.POINTER(C.c_float)]
+    L.ppf_solve_sweeps.argtypes = [vp, vp, vp, vp, vp, vp, vp, I, vp]
+    L.ppf_plan_tune.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(I)]
+    L.ppf_plan_set_zvariant.argtypes = [vp, I]
+    L.ppf_batch_create.argtypes = [I, C.POINTER(Desc), C.POINTER(BatchIO), I, C.POINTER(vp)]
+    L.ppf_batch_solve.argtypes = [vp, vp, C.POINTER(Stats)]
+    L.ppf_batch_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -209,6 +225,23 @@ class FFTPoisson:
         _check(lib().ppf_residual(self.handle, f.ptr, p.ptr, dmx3.ptr, dx3.ptr, stream, C.byref(r)), "ppf_residual")
         return r.value
 
+    def solve_sweeps(self, prhs: DeviceArray, p: DeviceArray, dmx1: DeviceArray, dmx2: DeviceArray,
+                     dmx3: DeviceArray, dx3: DeviceArray, sweeps: int, stream=None) -> None:
+        """Exactly `sweeps` refinement corrections, no convergence test (capture-safe)."""
+        _check(lib().ppf_solve_sweeps(self.handle, prhs.ptr, p.ptr, dmx1.ptr, dmx2.ptr, dmx3.ptr, dx3.ptr,
+                                      int(sweeps), stream), "ppf_solve_sweeps")
+
+    def tune(self, prhs: DeviceArray, p: DeviceArray, dmx1: DeviceArray, dmx2: DeviceArray, dmx3: DeviceArray,
+             dx3: DeviceArray, stream=None) -> int:
+        """Autotune the z-pass shape on this data; returns the chosen id."""
+        v = C.c_int(0)
+        _check(lib().ppf_plan_tune(self.handle, prhs.ptr, p.ptr, dmx1.ptr, dmx2.ptr, dmx3.ptr, dx3.ptr, stream,
+                                   C.byref(v)), "ppf_plan_tune")
+        return v.value
+
+    def set_zvariant(self, v: int) -> None:
+        _check(lib().ppf_plan_set_zvariant(self.handle, int(v)), "ppf_plan_set_zvariant")
+
     def clean(self) -> None:
         if self.handle:
             _check(lib().ppf_plan_destroy(self.handle), "ppf_plan_destroy")
@@ -217,6 +250,45 @@ class FFTPoisson:
     def __del__(self):
         try:
             self.clean()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent problems bound to their device buffers (ppf_batch_*)."""
+
+    def __init__(self, problems, flags: int = 0):
+        """problems: list of (FFTPoisson-style desc kwargs dict, Problem3D)."""
+        n = len(problems)
+        descs = (Desc * n)()
+        ios = (BatchIO * n)()
+        for i, (kw, dev) in enumerate(problems):
+            L = kw.get("L", (2.0, 2.0, 2.0))
+            pbc = kw.get("pbc", (False, False, False))
+            descs[i] = Desc(int(kw["n1m"]), int(kw["n2m"]), int(kw["n3m"]), float(L[0]), float(L[1]), float(L[2]),
+                            int(bool(pbc[0])), int(bool(pbc[1])), int(bool(pbc[2])),
+                            int(kw.get("precision", PPF_PREC_FP64)), float(kw.get("tol", 1e-10)),
+                            int(kw.get("max_sweeps", 8)))
+            ios[i] = BatchIO(dev.rhs.ptr, dev.p.ptr, dev.dmx1.ptr, dev.dmx2.ptr, dev.dmx3.ptr, dev.dx3.ptr)
+        h = C.c_void_p()
+        _check(lib().ppf_batch_create(n, descs, ios, int(flags), C.byref(h)), "ppf_batch_create")
+        self.handle = h.value
+        self.n = n
+
+    def solve(self, stream=None, stats: bool = False):
+        """Solve every problem; with stats, returns [(sweeps, rel_residual)] (synchronises)."""
+        st = (Stats * self.n)() if stats else None
+        _check(lib().ppf_batch_solve(self.handle, stream, st), "ppf_batch_solve")
+        return [(s.sweeps, s.rel_residual) for s in st] if stats else None
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib().ppf_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
         except Exception:
             pass
 
@@ -232,6 +304,11 @@ class TDMAPlanMany:
 
     def solve(self, a: DeviceArray, b: DeviceArray, c: DeviceArray, d: DeviceArray, stream=None) -> None:
         _check(lib().ppf_tdma_many_solve(self.handle, a.ptr, b.ptr, c.ptr, d.ptr, stream), "ppf_tdma_many_solve")
+
+    def solve_cycle(self, a: DeviceArray, b: DeviceArray, c: DeviceArray, d: DeviceArray, stream=None) -> None:
+        """PaScaL_TDMA_many_solve_cycle_cuda (one rank): cyclic systems."""
+        _check(lib().ppf_tdma_many_solve_cycle(self.handle, a.ptr, b.ptr, c.ptr, d.ptr, stream),
+               "ppf_tdma_many_solve_cycle")
 
     def destroy(self) -> None:
         if self.handle:

@@ -36,6 +36,34 @@ __device__ __forceinline__ cx<T> cconj(cx<T> a) { return {a.x, -a.y}; }
 template <typename T>
 __device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s}; }
 
+// ---- swizzled LDS lines ----------------------------------------------------
+// A Stockham stage writes its butterfly outputs R complex values apart
+// (R*16 bytes for FP64), which in a plain layout puts consecutive lanes on
+// the same LDS banks.  Complex index i is stored at i ^ ((i >> 3) & 7): the
+// low three bits are permuted by the next three, a bijection inside every
+// 8-value block, so 8 consecutive lanes at a stride of 1, 2, 4 or 8 values
+// land in 8 different 16-byte bank groups, at no extra LDS.
+__host__ __device__ constexpr int lds_pad(int i) { return i ^ ((i >> 3) & 7); }
+// slots for indices < m (an index past the last full block may move up to 7)
+__host__ __device__ constexpr int lds_slots(int m) { return ((m + 7) / 8) * 8; }
+
+// complex view of a swizzled line
+template <typename T>
+struct LLine {
+    cx<T> *p;
+    __device__ __forceinline__ cx<T> &operator[](int i) const { return p[lds_pad(i)]; }
+};
+// scalar view of the same line: scalar r is component r&1 of complex r>>1
+template <typename T>
+struct LReal {
+    T *p;
+    __device__ __forceinline__ T &operator[](int r) const { return p[2 * lds_pad(r >> 1) + (r & 1)]; }
+};
+template <typename T>
+__device__ __forceinline__ LReal<T> real_view(LLine<T> l) {
+    return LReal<T>{reinterpret_cast<T *>(l.p)};
+}
+
 // ---- compile-time radix plan -------------------------------------------
 constexpr bool fft_size_ok(int m) {
     if (m < 1) return false;
@@ -120,9 +148,9 @@ struct Dft<T, 8> {
 };
 
 // ---- one Stockham stage ---------------------------------------------------
-// line: LDS base of this thread's line (M complex), t: thread within the line.
+// line: this thread's LDS line (M complex, padded), t: thread within the line.
 template <typename T, int M, int TPL, int R, int LS>
-__device__ __forceinline__ void fft_stage(cx<T> *line, int t, const cx<T> *__restrict__ tw) {
+__device__ __forceinline__ void fft_stage(LLine<T> line, int t, const cx<T> *__restrict__ tw) {
     constexpr int S = M / R;                  // butterflies per line
     constexpr int NB = (S + TPL - 1) / TPL;   // butterflies per thread
     constexpr int STEP = M / (LS * R);        // twiddle table stride
@@ -156,7 +184,7 @@ __device__ __forceinline__ void fft_stage(cx<T> *line, int t, const cx<T> *__res
 
 template <typename T, int M, int TPL, int LS>
 struct FftStages {
-    __device__ __forceinline__ static void run(cx<T> *line, int t, const cx<T> *__restrict__ tw) {
+    __device__ __forceinline__ static void run(LLine<T> line, int t, const cx<T> *__restrict__ tw) {
         constexpr int R = pick_radix(M / LS);
         fft_stage<T, M, TPL, R, LS>(line, t, tw);
         FftStages<T, M, TPL, LS * R>::run(line, t, tw);
@@ -164,13 +192,13 @@ struct FftStages {
 };
 template <typename T, int M, int TPL>
 struct FftStages<T, M, TPL, M> {
-    __device__ __forceinline__ static void run(cx<T> *, int, const cx<T> *__restrict__) {}
+    __device__ __forceinline__ static void run(LLine<T>, int, const cx<T> *__restrict__) {}
 };
 
 // Forward FFT (sign -1) of one LDS line, in place, natural order in and out.
 // Every thread of the workgroup must call it (it contains barriers).
 template <typename T, int M, int TPL>
-__device__ __forceinline__ void fft_forward(cx<T> *line, int t, const cx<T> *__restrict__ tw) {
+__device__ __forceinline__ void fft_forward(LLine<T> line, int t, const cx<T> *__restrict__ tw) {
     static_assert(fft_size_ok(M), "FFT length must be 2^a or 3*2^a");
     FftStages<T, M, TPL, 1>::run(line, t, tw);
 }

@@ -21,6 +21,35 @@ namespace ppf {
 
 constexpr int kMaxNT = 512;
 
+// Streaming global accesses of the pass kernels: every element of a pass is
+// read once and written once, so PPF_NT_LOAD / PPF_NT_STORE (default on)
+// mark them nontemporal (no reuse expected in the caches).
+#ifndef PPF_NT_LOAD
+#define PPF_NT_LOAD 1
+#endif
+#ifndef PPF_NT_STORE
+#define PPF_NT_STORE 1
+#endif
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+    if constexpr (PPF_NT_LOAD) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <typename T>
+__device__ __forceinline__ void st_stream(T *p, T v) {
+    if constexpr (PPF_NT_STORE) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// Register budgets of the line-transform kernels, as minimum waves per SIMD
+// (occupancy is what hides the LDS and HBM latency of these kernels).
+#ifndef PPF_XWPE
+#define PPF_XWPE 3
+#endif
+#ifndef PPF_YWPE
+#define PPF_YWPE 1
+#endif
+
 // LDS layout of the line-transform kernels: NL lines of LSD scalars each
 // (LSD odd, so consecutive lines start on different banks and the
 // column-to-line scatter of the y passes is conflict-free), followed by the
@@ -28,11 +57,13 @@ constexpr int kMaxNT = 512;
 // together with the data so no FFT stage waits on a global round trip.
 template <typename T, int M, int S, int NL>
 struct LdsLines {
-    static constexpr int LSD = 2 * S + 1;
+    static constexpr int LSD = 2 * lds_slots(S) + 1;  // S complex values, padded
     static constexpr int TABN = M + 2 * (M + 1);
     static constexpr int SIZE = NL * LSD + 2 * TABN;  // scalars of type T
-    __device__ __forceinline__ static cx<T> *line(T *lds, int l) { return reinterpret_cast<cx<T> *>(lds + l * LSD); }
-    __device__ __forceinline__ static T *real(T *lds, int l) { return lds + l * LSD; }
+    __device__ __forceinline__ static LLine<T> line(T *lds, int l) {
+        return LLine<T>{reinterpret_cast<cx<T> *>(lds + l * LSD)};
+    }
+    __device__ __forceinline__ static LReal<T> real(T *lds, int l) { return LReal<T>{lds + l * LSD}; }
     __device__ __forceinline__ static cx<T> *tables(T *lds) { return reinterpret_cast<cx<T> *>(lds + NL * LSD); }
     __device__ __forceinline__ static void load_tables(T *lds, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
                                                        const cx<T> *__restrict__ qk, int tid, int nt) {
@@ -69,7 +100,7 @@ __device__ __forceinline__ int fdiv_small(int a, int b, float rb) {
 struct XRes {
     const double *P;          // (n1m+2)(n2m+2)(n3m+2) iterate
     const double *zam, *zap;  // z rows (zero at the walls)
-    double *partial;          // per-workgroup sums of r^2 and f^2
+    double *partial;          // per-workgroup sums of r^2, f^2 and x (3 per workgroup)
     double ih1, ih2;          // 1/h^2
     int pbc1, pbc2;
 };
@@ -78,7 +109,7 @@ struct XRes {
 // addresses are selected, never loaded values, so the seven loads issue
 // together.
 __device__ __forceinline__ double residual_at(const double *__restrict__ f, const XRes &r, int n, int j, int k, int N,
-                                              int n2m, double &fv) {
+                                              int n2m, double &fv, double &xv) {
     const size_t L1 = (size_t)N + 2;
     const size_t L2 = (size_t)n2m + 2;
     const size_t pc = ((size_t)(k + 1) * L2 + (size_t)(j + 1)) * L1 + (size_t)(n + 1);
@@ -102,6 +133,7 @@ __device__ __forceinline__ double residual_at(const double *__restrict__ f, cons
     const double zm = r.P[pc - pl], zp = r.P[pc + pl];  // ghost planes at the walls (finite, weight 0)
     const double am = r.zam[k], ap = r.zap[k];
     fv = f[((size_t)k * n2m + j) * N + n];
+    xv = c;
     const double lp = (xp - 2.0 * c + xm) * r.ih1 + (yp - 2.0 * c + ym) * r.ih2 + am * zm - (am + ap) * c + ap * zp;
     return fv - lp;
 }
@@ -131,9 +163,9 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 //
 // RES: the refinement form -- the transformed line is the FP64 residual
 // r = f - L x (in = f, xr.P = x) rather than in itself, and the workgroup's
-// sums of r^2 and f^2 go to xr.partial.
+// sums of r^2, f^2 and x go to xr.partial.
 template <typename T, int N, bool PER, bool RES = false>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RES ? 1 : PPF_XWPE)))
 k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, int Ly,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk, XRes xr) {
     using X = XGeom<T, N, PER>;
@@ -141,35 +173,40 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
     using L = typename X::L;
     constexpr int M = G::M, TPL = X::TPL, LPB = X::LPB, NT = X::NT;
     constexpr int LX = PER ? N + 2 : N;
-    constexpr int EPT = (LPB * N + NT - 1) / NT;  // loads per thread per tile
+    constexpr int H = N / 2, HX = LX / 2;          // sample pairs per line / per W row
+    constexpr int EPT = (LPB * H + NT - 1) / NT;  // pair loads per thread per tile
+    using D2 = double __attribute__((ext_vector_type(2)));
+    using V2 = T __attribute__((ext_vector_type(2)));
     __shared__ T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int ntiles = (nlines + LPB - 1) / LPB;
     const float rn2m = 1.0f / (float)n2m;
-    double pre[EPT];
-    double sr = 0.0, sf = 0.0;
-    // unconditional loads from clamped addresses (lines past the end are
-    // transformed but never stored); nothing consumes them before the next
-    // tile, so they stay in flight
+    D2 pre[EPT];
+    double sr = 0.0, sf = 0.0, sx = 0.0;
+    // unconditional 16-byte loads of sample pairs from clamped addresses
+    // (lines past the end are transformed but never stored); nothing
+    // consumes them before the next tile, so they stay in flight
     auto fetch = [&](int tile) {
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e0 = tid + q * NT;
-            const int e = e0 < LPB * N ? e0 : LPB * N - 1;
-            const int li = e / N, n = e - li * N;
+            const int e = e0 < LPB * H ? e0 : LPB * H - 1;
+            const int li = e / H, n = 2 * (e - li * H);
             const int line0 = tile * LPB + li;
             const int line = line0 < nlines ? line0 : nlines - 1;
             if constexpr (RES) {
                 const int k = fdiv_small(line, n2m, rn2m), j = line - k * n2m;
-                double fv;
-                const double r = residual_at(in, xr, n, j, k, N, n2m, fv);
-                pre[q] = r;
-                if (e0 < LPB * N && line0 < nlines) {
-                    sr += r * r;
-                    sf += fv * fv;
+                double fv0, xv0, fv1, xv1;
+                const double r0 = residual_at(in, xr, n, j, k, N, n2m, fv0, xv0);
+                const double r1 = residual_at(in, xr, n + 1, j, k, N, n2m, fv1, xv1);
+                pre[q] = D2{r0, r1};
+                if (e0 < LPB * H && line0 < nlines) {
+                    sr += r0 * r0 + r1 * r1;
+                    sf += fv0 * fv0 + fv1 * fv1;
+                    sx += xv0 + xv1;
                 }
             } else {
-                pre[q] = in[(size_t)line * N + n];
+                pre[q] = ld_stream(reinterpret_cast<const D2 *>(&in[(size_t)line * N + n]));
             }
         }
     };
@@ -177,15 +214,21 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
     if (tile < ntiles) fetch(tile);
     L::load_tables(lds, tw, wk, qk, tid, NT);
     const cx<T> *tab = L::tables(lds);
-    cx<T> *my = L::line(lds, l);
+    const LLine<T> my = L::line(lds, l);
     for (; tile < ntiles; tile += gridDim.x) {
         __syncthreads();  // the previous tile's LDS lines have been stored
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = tid + q * NT;
-            if (e < LPB * N) {
-                const int li = e / N, n = e - li * N;
-                L::real(lds, li)[PER ? n : makhoul_pos<N>(n)] = (T)pre[q];
+            if (e < LPB * H) {
+                const int li = e / H, h = e - li * H;
+                const D2 v = pre[q];
+                if (PER) {  // samples 2h, 2h+1 are complex h of the packed line
+                    L::line(lds, li)[h] = cx<T>{(T)v.x, (T)v.y};
+                } else {    // Makhoul: x_2h -> v_h, x_2h+1 -> v_{N-1-h}
+                    L::real(lds, li)[h] = (T)v.x;
+                    L::real(lds, li)[N - 1 - h] = (T)v.y;
+                }
             }
         }
         if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
@@ -195,13 +238,15 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
             r2c_post<T, N, TPL>(my, t, tab + M);
         else
             dct_post<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
-        for (int e = tid; e < LPB * LX; e += NT) {
-            const int li = e / LX, n = e - li * LX;
+        for (int e = tid; e < LPB * HX; e += NT) {
+            const int li = e / HX, h = e - li * HX;
             const int line = tile * LPB + li;
             if (line < nlines) {
                 // W row of prhs line (j,k): (k+1)*Ly + j + 1 = line + n2m + 3 + 2k
                 const int k = fdiv_small(line, n2m, rn2m);
-                W[((size_t)line + (size_t)(Ly + 1) + 2 * (size_t)k) * LX + n] = L::real(lds, li)[n];
+                const cx<T> z = L::line(lds, li)[h];
+                st_stream(reinterpret_cast<V2 *>(&W[((size_t)line + (size_t)(Ly + 1) + 2 * (size_t)k) * LX + 2 * h]),
+                          V2{z.x, z.y});
             }
         }
     }
@@ -209,9 +254,11 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
         __shared__ double red[NT];
         const double tr = block_sum<NT>(sr, red);
         const double tf = block_sum<NT>(sf, red);
+        const double tx = block_sum<NT>(sx, red);
         if (tid == 0) {
-            xr.partial[2 * blockIdx.x] = tr;
-            xr.partial[2 * blockIdx.x + 1] = tf;
+            xr.partial[3 * blockIdx.x] = tr;
+            xr.partial[3 * blockIdx.x + 1] = tf;
+            xr.partial[3 * blockIdx.x + 2] = tx;
         }
     }
 }
@@ -219,51 +266,60 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
 // ---------------------------------------------------------------------------
 // pass 5: x-inverse.  Every W row (Ly*Lz of them) -> one P row of N+2 with the
 // x ghosts; subtracts the mean found by the z pass.
-// ACC: the refinement form -- the row is added to P (x += e) instead of
-// stored; P's rows are prefetched with the W rows.
+// ACC: the refinement form -- the row is added to P (x += e - mean(e) -
+// mean(x), xsum[2] = sum of x's interior) instead of stored; P's rows are
+// prefetched with the W rows.
 // ---------------------------------------------------------------------------
 template <typename T, int N, bool PER, bool ACC = false>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ACC ? 1 : PPF_XWPE)))
 k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk,
-       const double *__restrict__ dmx1, const double *__restrict__ mean) {
+       const double *__restrict__ dmx1, const double *__restrict__ mean, const double *__restrict__ xsum,
+       double inv_cells) {
     using X = XGeom<T, N, PER>;
     using G = typename X::G;
     using L = typename X::L;
     constexpr int M = G::M, TPL = X::TPL, LPB = X::LPB, NT = X::NT;
     constexpr int LX = PER ? N + 2 : N;
     constexpr int LP = N + 2;
-    constexpr int EPT = (LPB * LX + NT - 1) / NT;
-    constexpr int EPTP = (LPB * LP + NT - 1) / NT;  // P elements stored per thread per tile
+    constexpr int HX = LX / 2, HP = LP / 2;  // scalar pairs per W row / per P row
+    constexpr int EPT = (LPB * HX + NT - 1) / NT;
+    constexpr int EPTP = (LPB * HP + NT - 1) / NT;  // P pairs stored per thread per tile
+    using V2 = T __attribute__((ext_vector_type(2)));
+    using D2 = double __attribute__((ext_vector_type(2)));
     __shared__ T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int ntiles = (nlines + LPB - 1) / LPB;
-    T pre[EPT];
-    double pold[ACC ? EPTP : 1];
+    V2 pre[EPT];
+    D2 pold[ACC ? EPTP : 1];
+    // 16-byte (FP64) pair loads of whole rows, clamped to the last pair
     auto fetch = [&](int tile) {
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             int e = tid + q * NT;
-            e = e < LPB * LX ? e : LPB * LX - 1;
-            size_t idx = (size_t)tile * LPB * LX + e;
-            const size_t last = (size_t)nlines * LX - 1;
-            pre[q] = W[idx < last ? idx : last];
+            e = e < LPB * HX ? e : LPB * HX - 1;
+            size_t idx = (size_t)tile * LPB * HX + e;
+            const size_t last = (size_t)nlines * HX - 1;
+            pre[q] = ld_stream(reinterpret_cast<const V2 *>(W) + (idx < last ? idx : last));
         }
         if constexpr (ACC) {
 #pragma unroll
             for (int q = 0; q < EPTP; ++q) {
                 int e = tid + q * NT;
-                e = e < LPB * LP ? e : LPB * LP - 1;
-                const size_t idx = (size_t)tile * LPB * LP + e;
-                const size_t last = (size_t)nlines * LP - 1;
-                pold[q] = P[idx < last ? idx : last];
+                e = e < LPB * HP ? e : LPB * HP - 1;
+                const size_t idx = (size_t)tile * LPB * HP + e;
+                const size_t last = (size_t)nlines * HP - 1;
+                pold[q] = ld_stream(reinterpret_cast<const D2 *>(P) + (idx < last ? idx : last));
             }
         }
     };
     int tile = blockIdx.x;
     if (tile < ntiles) fetch(tile);
     L::load_tables(lds, tw, wk, qk, tid, NT);
-    const double mu = *mean;
+    // ACC: the iterate's own mean (sum from the residual pass) goes too, so
+    // the refined solution keeps the zero mean exactly (the residual is blind
+    // to a constant, and FP32 storage of the (0,0) mode leaves one)
+    const double mu = *mean + (ACC ? xsum[2] * inv_cells : 0.0);
     double bca0 = 0, bcb0 = 0, bca1 = 0, bcb1 = 0;
     if (!PER) {
         const double d1 = dmx1[1], d2 = dmx1[2], e1 = dmx1[N + 1], e2 = dmx1[N];
@@ -276,18 +332,18 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
     }
     const double scale = PER ? 1.0 / (double)N : 1.0 / (2.0 * (double)N);
     const cx<T> *tab = L::tables(lds);
-    cx<T> *my = L::line(lds, l);
+    const LLine<T> my = L::line(lds, l);
     for (; tile < ntiles; tile += gridDim.x) {
         __syncthreads();  // the previous tile's LDS lines have been stored
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = tid + q * NT;
-            if (e < LPB * LX) {
-                const int li = e / LX, n = e - li * LX;
-                L::real(lds, li)[n] = pre[q];
+            if (e < LPB * HX) {
+                const int li = e / HX, h = e - li * HX;
+                L::line(lds, li)[h] = cx<T>{pre[q].x, pre[q].y};
             }
         }
-        double pcur[ACC ? EPTP : 1];
+        D2 pcur[ACC ? EPTP : 1];
         if constexpr (ACC) {
 #pragma unroll
             for (int q = 0; q < EPTP; ++q) pcur[q] = pold[q];
@@ -299,29 +355,38 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
         else
             idct_pre<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
         fft_forward<T, M, TPL>(my, t, tab);
-#pragma unroll
-        for (int q = 0; q < EPTP; ++q) {
-            const int e = tid + q * NT;
-            if (e >= LPB * LP) break;
-            const int li = e / LP, np = e - li * LP;
-            const int line = tile * LPB + li;
-            if (line >= nlines) continue;
-            const T *re = L::real(lds, li);
-            // physical sample n (0-based) of this line, mean removed
+        // P element e of the tile: physical sample of its line, mean removed,
+        // with the wall extrapolation or the wrap in the two x ghosts
+        auto value = [&](int li, int np) -> double {
+            const LReal<T> re = L::real(lds, li);
             auto sample = [&](int n) -> double {
                 const int tpos = PER ? n : makhoul_pos<N>(n);
                 const double v = (double)re[tpos];
                 return ((tpos & 1) ? -v : v) * scale - mu;
             };
-            double out;
-            if (np == 0)
-                out = PER ? sample(N - 1) : bca0 * sample(0) - bcb0 * sample(1);
-            else if (np == N + 1)
-                out = PER ? sample(0) : bca1 * sample(N - 1) - bcb1 * sample(N - 2);
-            else
-                out = sample(np - 1);
-            if constexpr (ACC) out += pcur[q];
-            P[(size_t)line * LP + np] = out;
+            if (np == 0) return PER ? sample(N - 1) : bca0 * sample(0) - bcb0 * sample(1);
+            if (np == N + 1) return PER ? sample(0) : bca1 * sample(N - 1) - bcb1 * sample(N - 2);
+            return sample(np - 1);
+        };
+        if constexpr (ACC) {
+#pragma unroll
+            for (int q = 0; q < EPTP; ++q) {
+                const int e = tid + q * NT;
+                if (e >= LPB * HP) break;
+                const int li = e / HP, h = e - li * HP;
+                const int line = tile * LPB + li;
+                if (line < nlines)
+                    st_stream(reinterpret_cast<D2 *>(&P[(size_t)line * LP + 2 * h]),
+                              D2{value(li, 2 * h), value(li, 2 * h + 1)} + pcur[q]);
+            }
+        } else {
+            for (int e = tid; e < LPB * HP; e += NT) {
+                const int li = e / HP, h = e - li * HP;
+                const int line = tile * LPB + li;
+                if (line < nlines)
+                    st_stream(reinterpret_cast<D2 *>(&P[(size_t)line * LP + 2 * h]),
+                              D2{value(li, 2 * h), value(li, 2 * h + 1)});
+            }
         }
     }
 }
@@ -340,7 +405,7 @@ struct YGeom {
     // scalar columns per tile: 128-byte rows, LDS (lines + tables) within
     // 96 KiB, threads within kMaxNT
     static constexpr int lds_bytes(int tx) {
-        return ((CPLX ? tx / 2 : tx) * (2 * S + 1) + 2 * (M + 2 * (M + 1))) * (int)sizeof(T);
+        return ((CPLX ? tx / 2 : tx) * (2 * lds_slots(S) + 1) + 2 * (M + 2 * (M + 1))) * (int)sizeof(T);
     }
     static constexpr int TX0 = 128 / (int)sizeof(T);
     static constexpr int tx_fit(int tx) {
@@ -353,36 +418,38 @@ struct YGeom {
 };
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(kMaxNT)
+__global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(PPF_YWPE)))
 k_yfwd(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
        const cx<T> *__restrict__ qk) {
     using Y = YGeom<T, N, MODE>;
     using L = typename Y::L;
     constexpr int M = Y::M, TPL = Y::TPL, TX = Y::TX, NT = Y::NT;
-    constexpr int EPT = (TX * N + NT - 1) / NT;
+    constexpr int TP = TX / 2;  // column pairs: global accesses are 2 adjacent scalars
+    constexpr int EPT = (TP * N + NT - 1) / NT;
+    using V2 = T __attribute__((ext_vector_type(2)));
     __shared__ T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int nct = (Lx + TX - 1) / TX;
     const int ntiles = nct * nplanes;
-    T pre[EPT];
+    V2 pre[EPT];
     auto fetch = [&](int tile) {  // unconditional, clamped: see k_xfwd
         const int ct = tile % nct, pl = tile / nct;
         const size_t plane = (size_t)(pl + 1) * Ly;
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             int e = tid + q * NT;
-            e = e < TX * N ? e : TX * N - 1;
-            const int r = e / TX, c = e - r * TX;
-            int col = ct * TX + c;
-            col = col < Lx ? col : Lx - 1;
-            pre[q] = W[(plane + 1 + r) * Lx + col];
+            e = e < TP * N ? e : TP * N - 1;
+            const int r = e / TP, c2 = e - r * TP;
+            int col = ct * TX + 2 * c2;
+            col = col < Lx ? col : Lx - 2;  // Lx is even
+            pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[(plane + 1 + r) * Lx + col]));
         }
     };
     int tile = blockIdx.x;
     if (tile < ntiles) fetch(tile);
     L::load_tables(lds, tw, wk, qk, tid, NT);
     const cx<T> *tab = L::tables(lds);
-    cx<T> *my = L::line(lds, l);
+    const LLine<T> my = L::line(lds, l);
     for (; tile < ntiles; tile += gridDim.x) {
     const int c0 = (tile % nct) * TX;
     const size_t plane = (size_t)(tile / nct + 1) * Ly;
@@ -390,15 +457,18 @@ k_yfwd(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int e = tid + q * NT;
-        if (e < TX * N) {
-            const int r = e / TX, c = e - r * TX;
-            const T v = pre[q];
-            if (MODE == 0)
-                L::real(lds, c)[makhoul_pos<N>(r)] = v;
-            else if (MODE == 1)
-                L::real(lds, c >> 1)[2 * r + (c & 1)] = v;
-            else
-                L::real(lds, c)[r] = v;
+        if (e < TP * N) {
+            const int r = e / TP, c2 = e - r * TP;
+            const V2 v = pre[q];
+            if (MODE == 0) {
+                L::real(lds, 2 * c2)[makhoul_pos<N>(r)] = v.x;
+                L::real(lds, 2 * c2 + 1)[makhoul_pos<N>(r)] = v.y;
+            } else if (MODE == 1) {  // a pair is one complex column
+                L::line(lds, c2)[r] = cx<T>{v.x, v.y};
+            } else {
+                L::real(lds, 2 * c2)[r] = v.x;
+                L::real(lds, 2 * c2 + 1)[r] = v.y;
+            }
         }
     }
     if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
@@ -408,22 +478,26 @@ k_yfwd(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
     if (MODE == 2) r2c_post<T, N, TPL>(my, t, tab + M);
     constexpr int NR = MODE == 2 ? N + 2 : N;  // spectral rows written
     constexpr int R0 = MODE == 2 ? 0 : 1;
-    for (int e = tid; e < TX * NR; e += NT) {
-        const int r = e / TX, c = e - r * TX;
-        const int col = c0 + c;
+    for (int e = tid; e < TP * NR; e += NT) {
+        const int r = e / TP, c2 = e - r * TP;
+        const int col = c0 + 2 * c2;
         if (col >= Lx) continue;
-        T v;
-        if (MODE == 1)
-            v = L::real(lds, c >> 1)[2 * r + (c & 1)];
-        else
-            v = L::real(lds, c)[r];
-        W[(plane + R0 + r) * Lx + col] = v;
+        V2 v;
+        if (MODE == 1) {
+            const cx<T> z = L::line(lds, c2)[r];
+            v.x = z.x;
+            v.y = z.y;
+        } else {
+            v.x = L::real(lds, 2 * c2)[r];
+            v.y = L::real(lds, 2 * c2 + 1)[r];
+        }
+        st_stream(reinterpret_cast<V2 *>(&W[(plane + R0 + r) * Lx + col]), v);
     }
     }
 }
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(kMaxNT)
+__global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(PPF_YWPE)))
 k_yinv(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
        const cx<T> *__restrict__ qk, const double *__restrict__ dmx2) {
     using Y = YGeom<T, N, MODE>;
@@ -431,23 +505,25 @@ k_yinv(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
     constexpr int M = Y::M, TPL = Y::TPL, TX = Y::TX, NT = Y::NT;
     constexpr int NR = MODE == 2 ? N + 2 : N;
     constexpr int R0 = MODE == 2 ? 0 : 1;
-    constexpr int EPT = (TX * NR + NT - 1) / NT;
+    constexpr int TP = TX / 2;  // column pairs: global accesses are 2 adjacent scalars
+    constexpr int EPT = (TP * NR + NT - 1) / NT;
+    using V2 = T __attribute__((ext_vector_type(2)));
     __shared__ T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int nct = (Lx + TX - 1) / TX;
     const int ntiles = nct * nplanes;  // every plane incl. the z ghost planes
-    T pre[EPT];
+    V2 pre[EPT];
     auto fetch = [&](int tile) {  // unconditional, clamped: see k_xfwd
         const int ct = tile % nct, pl = tile / nct;
         const size_t plane = (size_t)pl * Ly;
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             int e = tid + q * NT;
-            e = e < TX * NR ? e : TX * NR - 1;
-            const int r = e / TX, c = e - r * TX;
-            int col = ct * TX + c;
-            col = col < Lx ? col : Lx - 1;
-            pre[q] = W[(plane + R0 + r) * Lx + col];
+            e = e < TP * NR ? e : TP * NR - 1;
+            const int r = e / TP, c2 = e - r * TP;
+            int col = ct * TX + 2 * c2;
+            col = col < Lx ? col : Lx - 2;  // Lx is even
+            pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[(plane + R0 + r) * Lx + col]));
         }
     };
     int tile = blockIdx.x;
@@ -466,7 +542,7 @@ k_yinv(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
         bcb1 = (T)(e1 * e1 / (s1 * s1 - e1 * e1));
     }
     const cx<T> *tab = L::tables(lds);
-    cx<T> *my = L::line(lds, l);
+    const LLine<T> my = L::line(lds, l);
     for (; tile < ntiles; tile += gridDim.x) {
     const int c0 = (tile % nct) * TX;
     const size_t plane = (size_t)(tile / nct) * Ly;
@@ -474,13 +550,15 @@ k_yinv(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
         const int e = tid + q * NT;
-        if (e < TX * NR) {
-            const int r = e / TX, c = e - r * TX;
-            const T v = pre[q];
-            if (MODE == 1)  // conjugate on load (inverse via the forward FFT)
-                L::real(lds, c >> 1)[2 * r + (c & 1)] = (c & 1) ? -v : v;
-            else
-                L::real(lds, c)[r] = v;
+        if (e < TP * NR) {
+            const int r = e / TP, c2 = e - r * TP;
+            const V2 v = pre[q];
+            if (MODE == 1) {  // a complex column, conjugated (inverse via the forward FFT)
+                L::line(lds, c2)[r] = cx<T>{v.x, -v.y};
+            } else {
+                L::real(lds, 2 * c2)[r] = v.x;
+                L::real(lds, 2 * c2 + 1)[r] = v.y;
+            }
         }
     }
     if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
@@ -488,27 +566,33 @@ k_yinv(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
     if (MODE == 0) idct_pre<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
     if (MODE == 2) c2r_pre<T, N, TPL>(my, t, tab + M);
     fft_forward<T, M, TPL>(my, t, tab);
-    for (int e = tid; e < TX * (N + 2); e += NT) {
-        const int rp = e / TX, c = e - rp * TX;
-        const int col = c0 + c;
+    for (int e = tid; e < TP * (N + 2); e += NT) {
+        const int rp = e / TP, c2 = e - rp * TP;
+        const int col = c0 + 2 * c2;
         if (col >= Lx) continue;
-        auto sample = [&](int n) -> T {  // physical row n (0-based)
+        // physical row n (0-based) of the pair's two columns
+        auto sample = [&](int n) -> V2 {
+            V2 v;
             if (MODE == 1) {
-                const T v = L::real(lds, c >> 1)[2 * n + (c & 1)];
-                return ((c & 1) ? -v : v) * scale;
+                const cx<T> z = L::line(lds, c2)[n];
+                v.x = z.x * scale;
+                v.y = -z.y * scale;
+            } else {
+                const int tpos = MODE == 0 ? makhoul_pos<N>(n) : n;
+                const T sg = (tpos & 1) ? T(-1) : T(1);
+                v.x = sg * L::real(lds, 2 * c2)[tpos] * scale;
+                v.y = sg * L::real(lds, 2 * c2 + 1)[tpos] * scale;
             }
-            const int tpos = MODE == 0 ? makhoul_pos<N>(n) : n;
-            const T v = L::real(lds, c)[tpos];
-            return ((tpos & 1) ? -v : v) * scale;
+            return v;
         };
-        T out;
+        V2 out;
         if (rp == 0)
             out = MODE == 0 ? bca0 * sample(0) - bcb0 * sample(1) : sample(N - 1);
         else if (rp == N + 1)
             out = MODE == 0 ? bca1 * sample(N - 1) - bcb1 * sample(N - 2) : sample(0);
         else
             out = sample(rp - 1);
-        W[(plane + rp) * Lx + col] = out;
+        st_stream(reinterpret_cast<V2 *>(&W[(plane + rp) * Lx + col]), out);
     }
     }
 }
@@ -529,6 +613,7 @@ struct ZArgs {
     const unsigned char *zx, *zy;     // zero-mode flags
     int mean_kx, mean_jp;             // real (0,0) column
     double mean_scale;                // 1/(fx fy n1m n2m n3m)
+    int zper;                         // periodic z: ghost planes wrap instead of wall extrapolation
 };
 
 template <typename T>
@@ -600,8 +685,13 @@ k_zthomas(T *__restrict__ W, T *__restrict__ cs, ZArgs z, const double *__restri
     // now xk = x_1, xk1 = x_2; x_n and x_{n-1} are re-read (n >= 2)
     const double x1 = xk, x2 = xk1;
     const double xm1 = (double)w[(size_t)(n - 1) * pstride];
-    w[0] = (T)(bca0 * x1 - bcb0 * x2);
-    w[(size_t)(n + 1) * pstride] = (T)(bca1 * xn - bcb1 * xm1);
+    if (z.zper) {
+        w[0] = (T)xn;
+        w[(size_t)(n + 1) * pstride] = (T)x1;
+    } else {
+        w[0] = (T)(bca0 * x1 - bcb0 * x2);
+        w[(size_t)(n + 1) * pstride] = (T)(bca1 * xn - bcb1 * xm1);
+    }
     if (kx == z.mean_kx && jp == z.mean_jp) *mean_out = sum * z.mean_scale;
 }
 
@@ -689,7 +779,7 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
 #pragma unroll
     for (int l = 0; l < K; ++l) {
         const int g = g0 + l;
-        d[l] = (double)w[(size_t)((g < glast ? g : glast) + 1) * pstride];
+        d[l] = (double)ld_stream(&w[(size_t)((g < glast ? g : glast) + 1) * pstride]);
     }
 #pragma unroll
     for (int l = 0; l < K; ++l) d[l] = (active && g0 + l < n3m) ? d[l] : 0.0;
@@ -833,13 +923,17 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
         const int g = g0 + l;
         if (g < n3m) {
             part += x[l];
-            if (active) w[(size_t)(g + 1) * pstride] = (T)x[l];
+            if (active) st_stream(&w[(size_t)(g + 1) * pstride], (T)x[l]);
         }
     }
-    // spectral z ghost planes (wall extrapolation, fft_3d_poisson.f90:809-852)
+    // spectral z ghost planes: wall extrapolation (fft_3d_poisson.f90:809-852)
+    // or, for periodic z, the wrapped end planes (:666-796)
     if (active) {
-        if (p == 0) w[0] = (T)(bc[0] * x[0] - bc[1] * x[1]);
         const int gt = n3m - 1, pt = gt / K, lt = gt - pt * K;
+        if (p == 0) {
+            if (z.zper) w[(size_t)(n3m + 1) * pstride] = (T)x[0];
+            else w[0] = (T)(bc[0] * x[0] - bc[1] * x[1]);
+        }
         if (p == pt) {
             double xt = 0.0, xb = 0.0;
 #pragma unroll
@@ -848,7 +942,8 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
                 if (l == lt - 1) xb = x[l];
             }
             if (lt == 0) xb = rd[(2 * (pt - 1) + 1) * S + s];
-            w[(size_t)(n3m + 1) * pstride] = (T)(bc[2] * xt - bc[3] * xb);
+            if (z.zper) w[0] = (T)xt;
+            else w[(size_t)(n3m + 1) * pstride] = (T)(bc[2] * xt - bc[3] * xb);
         }
     }
     // mean of the solution from the (0,0) real column (:628-664)
@@ -1043,7 +1138,7 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
             const int g = g0 + l;
             if (g < n3m) {
                 part += x[l];
-                if (active) w[(size_t)(g + 1) * pstride] = (T)x[l];
+                if (active) st_stream(&w[(size_t)(g + 1) * pstride], (T)x[l]);
             }
         }
         if (active) {
@@ -1076,52 +1171,85 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
 
 // ---------------------------------------------------------------------------
 // FP64 residual r = f - L p on interior cells of the ghosted P (the operator
-// of residual_at), with per-workgroup sums of r^2 and f^2 (deterministic
-// two-level reduction with k_reduce_pairs).  Workgroups stride over the
+// of residual_at), with per-workgroup sums of r^2, f^2 and p (deterministic
+// two-level reduction with k_reduce3).  Workgroups stride over the
 // n2m*n3m rows; threads over a row.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m) {
     __shared__ double red[256];
-    double sr = 0.0, sf = 0.0;
+    double sr = 0.0, sf = 0.0, sx = 0.0;
     const int nrows = n2m * n3m;
     for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
         const int k = row / n2m, j = row - k * n2m;
         for (int i = threadIdx.x; i < n1m; i += blockDim.x) {
-            double fv;
-            const double r = residual_at(f, xr, i, j, k, n1m, n2m, fv);
+            double fv, xv;
+            const double r = residual_at(f, xr, i, j, k, n1m, n2m, fv, xv);
             sr += r * r;
             sf += fv * fv;
+            sx += xv;
         }
     }
     const double tr = block_sum<256>(sr, red);
     const double tf = block_sum<256>(sf, red);
+    const double tx = block_sum<256>(sx, red);
     if (threadIdx.x == 0) {
-        xr.partial[2 * blockIdx.x] = tr;
-        xr.partial[2 * blockIdx.x + 1] = tf;
+        xr.partial[3 * blockIdx.x] = tr;
+        xr.partial[3 * blockIdx.x + 1] = tf;
+        xr.partial[3 * blockIdx.x + 2] = tx;
     }
 }
 
-__global__ void __launch_bounds__(256) k_reduce_pairs(const double *__restrict__ partial, int n, double *__restrict__ out) {
-    __shared__ double s_r[256], s_f[256];
-    double a = 0.0, b = 0.0;
+// Totals of the per-workgroup triples (sum r^2, sum f^2, sum x) in a fixed
+// order (deterministic).
+__global__ void __launch_bounds__(256) k_reduce3(const double *__restrict__ partial, int n, double *__restrict__ out) {
+    __shared__ double red[256];
+    double a = 0.0, b = 0.0, c = 0.0;
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        a += partial[2 * q];
-        b += partial[2 * q + 1];
+        a += partial[3 * q];
+        b += partial[3 * q + 1];
+        c += partial[3 * q + 2];
     }
-    s_r[threadIdx.x] = a;
-    s_f[threadIdx.x] = b;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            s_r[threadIdx.x] += s_r[threadIdx.x + s];
-            s_f[threadIdx.x] += s_f[threadIdx.x + s];
-        }
-        __syncthreads();
-    }
+    a = block_sum<256>(a, red);
+    b = block_sum<256>(b, red);
+    c = block_sum<256>(c, red);
     if (threadIdx.x == 0) {
-        out[0] = s_r[0];
-        out[1] = s_f[0];
+        out[0] = a;
+        out[1] = b;
+        out[2] = c;
+    }
+}
+
+// Ghost layers of P from its interior, one face pair per launch:
+// wall extrapolation p0 = BC_a p1 - BC_b p2 (cuda_neumann_BC /
+// apply_bc_on_direction, fft_3d_poisson.f90:798-852) or periodic wrap
+// (cuda_ghostcell_update on one rank, :666-796), over the full (ghosted)
+// ranges of the other two axes.  The reference's order -- walls along x, y,
+// z, then wraps along x, y, z -- is the caller's launch order.
+__global__ void __launch_bounds__(256)
+k_ghost_face(double *__restrict__ P, int n1m, int n2m, int n3m, int dir, int wrap, const double *__restrict__ dmx) {
+    const size_t L1 = (size_t)n1m + 2, L2 = (size_t)n2m + 2, L3 = (size_t)n3m + 2;
+    const size_t nb = dir == 0 ? L2 : L1, nc = dir == 2 ? L2 : L3;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nb * nc) return;
+    const size_t b = q % nb, c = q / nb;
+    const int nm = dir == 0 ? n1m : (dir == 1 ? n2m : n3m);
+    // element (b, c, position along dir)
+    size_t base, step;
+    if (dir == 0) { base = (c * L2 + b) * L1; step = 1; }
+    else if (dir == 1) { base = c * L1 * L2 + b; step = L1; }
+    else { base = c * L1 + b; step = L1 * L2; }
+    double *x = P + base;
+    if (wrap) {
+        const double lo = x[1 * step], hi = x[(size_t)nm * step];
+        x[0] = hi;
+        x[(size_t)(nm + 1) * step] = lo;
+    } else {
+        const double d1 = dmx[1], d2 = dmx[2], s0 = d1 + d2;
+        const double e1 = dmx[nm + 1], e2 = dmx[nm], s1 = e1 + e2;
+        x[0] = (s0 * s0 / (s0 * s0 - d1 * d1)) * x[step] - (d1 * d1 / (s0 * s0 - d1 * d1)) * x[2 * step];
+        x[(size_t)(nm + 1) * step] = (s1 * s1 / (s1 * s1 - e1 * e1)) * x[(size_t)nm * step] -
+                                     (e1 * e1 / (s1 * s1 - e1 * e1)) * x[(size_t)(nm - 1) * step];
     }
 }
 

@@ -56,7 +56,7 @@ __device__ __forceinline__ int makhoul_pos(int n) {
 // Pair (k, M-k) is handled by one thread.  Result written back as V[0..M]
 // (M+1 complex).  Barrier inside.
 template <typename T, int N, int TPL>
-__device__ __forceinline__ void r2c_post(cx<T> *line, int t, const cx<T> *__restrict__ wk) {
+__device__ __forceinline__ void r2c_post(LLine<T> line, int t, const cx<T> *__restrict__ wk) {
     constexpr int M = N / 2;
     constexpr int NP = M / 2 + 1;  // pairs k = 0..M/2
     constexpr int NB = (NP + TPL - 1) / TPL;
@@ -97,7 +97,7 @@ __device__ __forceinline__ void r2c_post(cx<T> *line, int t, const cx<T> *__rest
 // DCT-II post: from Z (FFT of packed Makhoul sequence) to X[0..N-1] written as
 // the real view of the line.  Barrier inside.
 template <typename T, int N, int TPL>
-__device__ __forceinline__ void dct_post(cx<T> *line, int t, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk) {
+__device__ __forceinline__ void dct_post(LLine<T> line, int t, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk) {
     constexpr int M = N / 2;
     constexpr int NP = M / 2 + 1;
     constexpr int NB = (NP + TPL - 1) / TPL;
@@ -111,7 +111,7 @@ __device__ __forceinline__ void dct_post(cx<T> *line, int t, const cx<T> *__rest
         }
     }
     __syncthreads();
-    T *re = reinterpret_cast<T *>(line);
+    const LReal<T> re = real_view(line);
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int k = t + q * TPL;
@@ -147,7 +147,7 @@ __device__ __forceinline__ void dct_post(cx<T> *line, int t, const cx<T> *__rest
 // so that a forward FFT followed by conjugation yields the unnormalised
 // inverse z (z_m = v_{2m} + i v_{2m+1}).  Im V_0 and Im V_M are ignored.
 template <typename T, int N, int TPL>
-__device__ __forceinline__ void c2r_pre(cx<T> *line, int t, const cx<T> *__restrict__ wk) {
+__device__ __forceinline__ void c2r_pre(LLine<T> line, int t, const cx<T> *__restrict__ wk) {
     constexpr int M = N / 2;
     constexpr int NP = M / 2 + 1;
     constexpr int NB = (NP + TPL - 1) / TPL;
@@ -191,11 +191,11 @@ __device__ __forceinline__ void c2r_pre(cx<T> *line, int t, const cx<T> *__restr
 // DCT inverse pre: from X[0..N-1] (real view of the line) to conj(Y) with
 // V_k = exp(i pi k/2N)(X_k - i X_{N-k}), X_N = 0.  Barrier inside.
 template <typename T, int N, int TPL>
-__device__ __forceinline__ void idct_pre(cx<T> *line, int t, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk) {
+__device__ __forceinline__ void idct_pre(LLine<T> line, int t, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk) {
     constexpr int M = N / 2;
     constexpr int NP = M / 2 + 1;
     constexpr int NB = (NP + TPL - 1) / TPL;
-    const T *re = reinterpret_cast<const T *>(line);
+    const LReal<T> re = real_view(line);
     T x0[NB], x1[NB], x2[NB], x3[NB];  // X_k, X_{N-k}, X_{M-k}, X_{M+k}
 #pragma unroll
     for (int q = 0; q < NB; ++q) {

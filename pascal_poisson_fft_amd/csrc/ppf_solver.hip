@@ -63,7 +63,7 @@ struct ppf_plan {
     unsigned char *zx = nullptr, *zy = nullptr;
     double *mean = nullptr;     // device scalar
     double *partial = nullptr;  // residual partial sums
-    double *red = nullptr;      // residual totals (2)
+    double *red = nullptr;      // residual totals: sum r^2, sum f^2, sum p (3)
     int npartial = 0;
     double *hred = nullptr;     // pinned host copy of red (refinement convergence test)
     AxisTables<double> tx64, ty64;
@@ -178,8 +178,9 @@ static void launch_xinv_k(const ppf_plan *p, const T *W, double *P, const AxisTa
     using X = XGeom<T, N, PER>;
     const int nlines = p->Ly * p->Lz;
     const unsigned g = persistent_grid(k_xinv<T, N, PER, ACC>, X::NT, (nlines + X::LPB - 1) / X::LPB);
+    const double inv_cells = 1.0 / ((double)p->d.n1m * (double)p->d.n2m * (double)p->d.n3m);
     hipLaunchKernelGGL((k_xinv<T, N, PER, ACC>), dim3(g), dim3(X::NT), 0, s, W, P, nlines, tb.tw, tb.wk, tb.qk, dmx1,
-                       p->mean);
+                       p->mean, p->red, inv_cells);
 }
 
 template <typename T, int N>
@@ -271,6 +272,7 @@ static int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hi
     z.mean_kx = p->mean_kx;
     z.mean_jp = p->mean_jp;
     z.mean_scale = p->mean_scale;
+    z.zper = p->d.pbc3 != 0;
     const int n3m = p->d.n3m;
     // z row coefficients of this solve (also read by the refinement residual)
     hipLaunchKernelGGL(k_zcoef, dim3((n3m + 255) / 256), dim3(256), 0, s, dx3, dmx3, n3m, p->zcoef, p->zcoef + n3m);
@@ -306,6 +308,8 @@ static int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hi
     int v = p->zvariant;
     // measured on MI355X at 512^3: (K=32, S=16) 0.54 ms, pipelined K=16 0.81 ms
     if (v == 0) v = (n3m <= 1024) ? 3 : 4;
+    // the pipelined form writes wall ghost planes only
+    if (z.zper && v >= 7) v = (n3m <= 1024) ? 3 : 4;
     switch (v) {
         case 7: pipe(integral_constant<int, 16>{}, integral_constant<int, 16>{}, integral_constant<int, 2>{}); break;
         case 8: pipe(integral_constant<int, 16>{}, integral_constant<int, 8>{}, integral_constant<int, 2>{}); break;
@@ -375,7 +379,6 @@ int ppf_plan_create(const ppf_desc *desc, ppf_plan **out) {
     // z rows live in LDS in the z pass (2 doubles per row): n3m <= 4096
     if (!size_supported(d.n1m) || !size_supported(d.n2m) || d.n3m < 2 || (d.n3m & 1) || d.n3m > 4096)
         return PPF_ERR_SIZE;
-    if (d.pbc3) return PPF_ERR_UNSUPPORTED;  // cyclic z Thomas not provided
 
     ppf_plan *p = new (std::nothrow) ppf_plan();
     if (!p) return PPF_ERR_ALLOC;
@@ -407,7 +410,7 @@ int ppf_plan_create(const ppf_desc *desc, ppf_plan **out) {
     p->cs_elems = csz;
     if ((rc = fp32 ? dev_alloc(p, &p->W32, wsz) : dev_alloc(p, &p->W, wsz)) || (rc = dev_alloc(p, &p->cs, csz)) || (rc = dev_alloc(p, &p->lamx, p->Lx)) ||
         (rc = dev_alloc(p, &p->lamy, p->Ly)) || (rc = dev_alloc(p, &p->zx, p->Lx)) || (rc = dev_alloc(p, &p->zy, p->Ly)) ||
-        (rc = dev_alloc(p, &p->mean, 1)) || (rc = dev_alloc(p, &p->red, 2)) ||
+        (rc = dev_alloc(p, &p->mean, 1)) || (rc = dev_alloc(p, &p->red, 3)) ||
         (rc = dev_alloc(p, &p->zcoef, 2 * (size_t)d.n3m + 4))) {
         free_plan(p);
         return rc;
@@ -415,8 +418,8 @@ int ppf_plan_create(const ppf_desc *desc, ppf_plan **out) {
     // partial sums: one pair per workgroup of the residual kernels (persistent
     // grids are capped at this count)
     p->npartial = 4096;
-    if ((rc = dev_alloc(p, &p->partial, 2 * (size_t)p->npartial))) { free_plan(p); return rc; }
-    if (hipHostMalloc((void **)&p->hred, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    if ((rc = dev_alloc(p, &p->partial, 3 * (size_t)p->npartial))) { free_plan(p); return rc; }
+    if (hipHostMalloc((void **)&p->hred, 3 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         p->hred = nullptr;
         free_plan(p);
         return PPF_ERR_ALLOC;
@@ -511,10 +514,32 @@ static XRes make_xres(const ppf_plan *p, const double *P) {
 // pass 5) until ||r||/||f|| <= tol or max_sweeps corrections.  The z pass
 // computes in FP64 on FP32 data, so the pinned (0,0) system and the mean stay
 // FP64.  tol < 0: exactly max_sweeps corrections, no convergence test.
+// Ghost layers of P re-derived from its interior, in the reference's order
+// (walls x, y, z, then wraps x, y, z; fft_3d_poisson.f90:620-622).
+static int refresh_ghosts(const ppf_plan *p, double *P, const double *dmx1, const double *dmx2, const double *dmx3,
+                          hipStream_t s) {
+    const int n[3] = {p->d.n1m, p->d.n2m, p->d.n3m};
+    const int per[3] = {p->d.pbc1 != 0, p->d.pbc2 != 0, p->d.pbc3 != 0};
+    const double *dmx[3] = {dmx1, dmx2, dmx3};
+    for (int wrap = 0; wrap < 2; ++wrap)
+        for (int dir = 0; dir < 3; ++dir) {
+            if (per[dir] != wrap) continue;
+            const size_t nb = (size_t)(dir == 0 ? n[1] : n[0]) + 2, nc = (size_t)(dir == 2 ? n[1] : n[2]) + 2;
+            const unsigned grid = (unsigned)((nb * nc + 255) / 256);
+            hipLaunchKernelGGL(k_ghost_face, dim3(grid), dim3(256), 0, s, P, n[0], n[1], n[2], dir, wrap, dmx[dir]);
+        }
+    PPF_HIP(hipGetLastError());
+    return PPF_OK;
+}
+
+// fixed >= 0 overrides the plan: exactly `fixed` corrections, no test.
+// The ghost layers are accumulated with each correction, which carries the
+// FP32 rounding of the first solve's ghosts along (the residual never sees
+// them), so they are re-derived from the refined interior at the end.
 static int solve_fp32_ir(ppf_plan *p, const double *f, double *P, const double *dmx1, const double *dmx2,
-                         const double *dmx3, const double *dx3, hipStream_t s, ppf_stats *stats) {
-    const double tol = p->d.tol == 0.0 ? 1e-10 : p->d.tol;
-    const int maxs = p->d.max_sweeps > 0 ? p->d.max_sweeps : 8;
+                         const double *dmx3, const double *dx3, hipStream_t s, ppf_stats *stats, int fixed = -1) {
+    const double tol = fixed >= 0 ? -1.0 : (p->d.tol == 0.0 ? 1e-10 : p->d.tol);
+    const int maxs = fixed >= 0 ? fixed : (p->d.max_sweeps > 0 ? p->d.max_sweeps : 8);
     int rc;
     if ((rc = run_passes<float>(p, p->W32, p->tx32, p->ty32, f, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, nullptr,
                                 nullptr)))
@@ -524,10 +549,18 @@ static int solve_fp32_ir(ppf_plan *p, const double *f, double *P, const double *
     double rel = -1.0;
     bool converged = false;
     if (tol < 0) {
-        for (; sweeps < maxs; ++sweeps)
-            if ((rc = run_passes<float>(p, p->W32, p->tx32, p->ty32, f, &xr, P, true, dmx1, dmx2, dmx3, dx3, s,
-                                        nullptr, nullptr)))
+        for (; sweeps < maxs; ++sweeps) {
+            unsigned g = 0;
+            if ((rc = dispatch_xfwd<float>(p, f, p->W32, p->tx32, &xr, s, &g))) return rc;
+            hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, s, p->partial, (int)g, p->red);
+            if ((rc = dispatch_y<float>(p, p->W32, p->ty32, false, dmx2, s)) ||
+                (rc = launch_z<float>(p, p->W32, dx3, dmx3, s)) ||
+                (rc = dispatch_y<float>(p, p->W32, p->ty32, true, dmx2, s)) ||
+                (rc = dispatch_xinv<float>(p, p->W32, P, p->tx32, dmx1, true, s)))
                 return rc;
+            PPF_HIP(hipGetLastError());
+        }
+        if (sweeps > 0 && (rc = refresh_ghosts(p, P, dmx1, dmx2, dmx3, s))) return rc;
         if (stats) {
             if ((rc = ppf_residual(p, f, P, dmx3, dx3, s, &rel))) return rc;
         }
@@ -537,8 +570,8 @@ static int solve_fp32_ir(ppf_plan *p, const double *f, double *P, const double *
             // residual of the current iterate, transformed by pass 1
             unsigned g = 0;
             if ((rc = dispatch_xfwd<float>(p, f, p->W32, p->tx32, &xr, s, &g))) return rc;
-            hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(256), 0, s, p->partial, (int)g, p->red);
-            PPF_HIP(hipMemcpyAsync(p->hred, p->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+            hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, s, p->partial, (int)g, p->red);
+            PPF_HIP(hipMemcpyAsync(p->hred, p->red, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
             PPF_HIP(hipStreamSynchronize(s));
             rel = p->hred[1] > 0 ? std::sqrt(p->hred[0] / p->hred[1]) : std::sqrt(p->hred[0]);
             if (rel <= tol) { converged = true; break; }
@@ -551,6 +584,7 @@ static int solve_fp32_ir(ppf_plan *p, const double *f, double *P, const double *
             PPF_HIP(hipGetLastError());
             ++sweeps;
         }
+        if (sweeps > 0 && (rc = refresh_ghosts(p, P, dmx1, dmx2, dmx3, s))) return rc;
     }
     if (stats) {
         stats->sweeps = sweeps;
@@ -571,6 +605,48 @@ int ppf_solve(ppf_plan *p, const double *prhs, double *P, const double *dmx1, co
                                  nullptr, nullptr)))
         return rc;
     if (stats) { stats->sweeps = 0; stats->rel_residual = -1.0; }
+    return PPF_OK;
+}
+
+int ppf_solve_sweeps(ppf_plan *p, const double *prhs, double *P, const double *dmx1, const double *dmx2,
+                     const double *dmx3, const double *dx3, int sweeps, void *stream) {
+    if (!p || !prhs || !P || !dmx1 || !dmx2 || !dmx3 || !dx3 || sweeps < 0) return PPF_ERR_INVALID_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (p->d.precision == PPF_PREC_FP32_IR) return solve_fp32_ir(p, prhs, P, dmx1, dmx2, dmx3, dx3, s, nullptr, sweeps);
+    if (sweeps != 0) return PPF_ERR_INVALID_ARG;
+    return run_passes<double>(p, p->W, p->tx64, p->ty64, prhs, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, nullptr,
+                              nullptr);
+}
+
+int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1, const double *dmx2,
+                  const double *dmx3, const double *dx3, void *stream, int *zvariant) {
+    if (!p || !prhs || !P || !dmx1 || !dmx2 || !dmx3 || !dx3) return PPF_ERR_INVALID_ARG;
+    // candidates: partitioned (K,S) shapes and the pipelined wavefront-PCR form
+    static const int cand[] = {3, 1, 2, 4, 5, 7, 8};
+    const int keep = p->zvariant;
+    int best = 0;
+    float best_ms = 0.f;
+    for (int v : cand) {
+        p->zvariant = v;
+        float st[5];
+        float z = 0.f;
+        bool ok = true;
+        for (int rep = 0; rep < 3 && ok; ++rep) {
+            const int rc = ppf_solve_timed(p, prhs, P, dmx1, dmx2, dmx3, dx3, stream, st);
+            if (rc == PPF_ERR_UNSUPPORTED) ok = false;
+            else if (rc) { p->zvariant = keep; return rc; }
+            else if (rep > 0) z += st[2];
+        }
+        if (ok && (best == 0 || z < best_ms)) { best = v; best_ms = z; }
+    }
+    p->zvariant = best ? best : keep;
+    if (zvariant) *zvariant = p->zvariant;
+    return PPF_OK;
+}
+
+int ppf_plan_set_zvariant(ppf_plan *p, int zvariant) {
+    if (!p || zvariant < 0 || zvariant > 11) return PPF_ERR_INVALID_ARG;
+    p->zvariant = zvariant;
     return PPF_OK;
 }
 
@@ -610,9 +686,9 @@ int ppf_residual(ppf_plan *p, const double *f, const double *P, const double *dm
     const XRes xr = make_xres(p, P);
     const int g = std::min(p->npartial, p->d.n2m * n3m);
     hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, s, f, xr, p->d.n1m, p->d.n2m, n3m);
-    hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(256), 0, s, p->partial, g, p->red);
+    hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, s, p->partial, g, p->red);
     PPF_HIP(hipGetLastError());
-    PPF_HIP(hipMemcpyAsync(p->hred, p->red, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    PPF_HIP(hipMemcpyAsync(p->hred, p->red, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
     PPF_HIP(hipStreamSynchronize(s));
     *rel = p->hred[1] > 0 ? std::sqrt(p->hred[0] / p->hred[1]) : std::sqrt(p->hred[0]);
     return PPF_OK;

@@ -1,0 +1,100 @@
+"""GPU checks of the batched solve (ppf_batch_*) and the z-pass autotuner.
+
+A batch of problems with uneven extents, mixed BCs, uniform/tanh z, FP64 and
+FP32_IR. The first solve runs eagerly (and learns the refinement sweeps); later
+solves replay one hipGraph. Each replay is checked against the eager result and
+against the oracle. Eager host work (zeroing P, synchronisation) runs between
+replays.
+
+Tolerances: FP64 problems match the oracle to <= 1e-12 relative L2. FP32_IR
+problems reach the relative residual 1e-10 and are <= 1e-8 from the oracle.
+A graph replay reproduces the eager result bitwise: it runs the same kernels on
+the same data.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pascal_poisson_fft_amd import api
+from pascal_poisson_fft_amd import workloads as W
+from pascal_poisson_fft_amd.grid import make_axis
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [  # n1, n2, n3, pbc1, pbc2, uniform z, gamma, precision
+    (32, 48, 40, True, True, True, 0.0, api.PPF_PREC_FP32_IR),
+    (64, 32, 72, False, False, False, 1.4, api.PPF_PREC_FP32_IR),
+    (48, 64, 33 * 2, True, False, False, 1.9, api.PPF_PREC_FP64),
+    (96, 24, 128, False, True, True, 0.0, api.PPF_PREC_FP64),
+    (128, 128, 32, False, False, False, 1.0, api.PPF_PREC_FP32_IR),
+    (32, 48, 40, True, True, True, 0.0, api.PPF_PREC_FP32_IR),  # same shape as problem 0: shares its tuning
+]
+
+
+def _problems(seed0=40):
+    out = []
+    for i, (n1, n2, n3, p1, p2, uz, g, prec) in enumerate(SHAPES):
+        L = (1.0, 1.0, 1.0)
+        ax = make_axis(n1, L[0], p1)
+        ay = make_axis(n2, L[1], p2)
+        az = make_axis(n3, L[2], False, uniform=uz, gamma=g)
+        f = W.remove_weighted_mean(np.random.default_rng(seed0 + i).standard_normal((n3, n2, n1)), az)
+        dev = api.Problem3D(f, ax, ay, az)
+        kw = dict(n1m=n1, n2m=n2, n3m=n3, L=L, pbc=(p1, p2, False), precision=prec, tol=1e-10, max_sweeps=8)
+        ref = O.solve(f, L, (p1, p2, False), ax.dmx, ay.dmx, az.dmx, az.dx)
+        out.append((kw, dev, ref))
+    return out
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("flags", [0, api.PPF_BATCH_NO_GRAPH, api.PPF_BATCH_NO_TUNE])
+def test_batch_eager_then_graph_replays(flags):
+    probs = _problems()
+    print(f"batch flags={flags}: {len(probs)} problems", flush=True)
+    b = api.Batch([(kw, dev) for kw, dev, _ in probs], flags=flags)
+    try:
+        st = b.solve(stats=True)
+        eager = [dev.p.download() for _, dev, _ in probs]
+        for (kw, _, ref), P, (sw, res) in zip(probs, eager, st):
+            if kw["precision"] == api.PPF_PREC_FP64:
+                assert sw == 0 and _rel(P, ref) <= 1e-12
+            else:
+                assert 1 <= sw <= 5 and res <= 1e-10 and _rel(P, ref) <= 1e-8
+        for rep in range(3):
+            for _, dev, _ in probs:
+                dev.p.fill_zero()
+            api.sync()
+            b.solve()
+            api.sync()
+            for (kw, dev, ref), P0 in zip(probs, eager):
+                P = dev.p.download()
+                assert np.array_equal(P, P0), f"replay {rep} differs from the eager solve"
+        st2 = b.solve(stats=True)
+        for (kw, _, _), (sw, res) in zip(probs, st2):
+            if kw["precision"] == api.PPF_PREC_FP32_IR:
+                assert res <= 1e-10
+    finally:
+        b.destroy()
+        for _, dev, _ in probs:
+            dev.free()
+
+
+def test_tune_picks_a_variant_and_keeps_parity():
+    prob = W.smoothed_normal((64, 64, 96), (False, False, False), 1.5, seed=9)
+    ax, ay, az = prob.ax
+    plan = api.FFTPoisson(64, 64, 96, prob.L, prob.pbc)
+    dev = api.Problem3D(prob.rhs, ax, ay, az)
+    try:
+        args = (dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3)
+        v = plan.tune(*args)
+        assert v > 0
+        plan.solve(*args)
+        api.sync()
+        ref = O.solve(prob.rhs, prob.L, prob.pbc, ax.dmx, ay.dmx, az.dmx, az.dx)
+        assert _rel(dev.p.download(), ref) <= 1e-12
+    finally:
+        dev.free()
+        plan.clean()

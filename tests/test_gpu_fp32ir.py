@@ -79,7 +79,8 @@ def test_fp32_ir_fixed_sweeps_mode():
     assert res3 == pytest.approx(st3.rel_residual, rel=1e-6)
     P0, st0, res0 = _run(ax, ay, az, f, tol=-1.0, max_sweeps=0)  # <= 0: the default cap 8
     assert st0.sweeps == 8
-    assert res3 < 1e-9 and res0 <= res3 * 1.01
+    # both are at the FP64 rounding floor of the residual, so only bound them
+    assert res3 < 1e-12 and res0 < 1e-12
 
 
 def test_fp32_ir_reports_not_converged():

@@ -133,5 +133,38 @@ def test_plan_rejects_bad_sizes():
         api.FFTPoisson(8, 8, 7)  # odd z
     assert e.value.code == -2
     with pytest.raises(api.PPFError) as e:
-        api.FFTPoisson(8, 8, 8, pbc=(False, False, True))
-    assert e.value.code == -5
+        api.FFTPoisson(8, 8, 4096 + 2)  # z lines beyond the supported 4096 rows
+    assert e.value.code == -2
+
+
+@pytest.mark.parametrize("pbc", BCS, ids=["NN", "PP", "PN", "NP"])
+@pytest.mark.parametrize("n1,n2,n3", [(16, 12, 10), (64, 32, 48)])
+def test_periodic_z_matches_oracle(pbc, n1, n2, n3):
+    """pbc3: the reference's z rows (walls' am/ap zeroing kept) with wrapped z
+    ghost planes, on a periodic z grid."""
+    L = (2.0, 3.0, 1.5)
+    ax = make_axis(n1, L[0], pbc[0])
+    ay = make_axis(n2, L[1], pbc[1])
+    az = make_axis(n3, L[2], True)
+    f = np.random.default_rng(n1 + n2 + n3).standard_normal((n3, n2, n1))
+    f -= f.mean()
+    got = api.solve_host(f, ax, ay, az)
+    ref = O.solve(f, L, (pbc[0], pbc[1], True), ax.dmx, ay.dmx, az.dmx, az.dx)
+    assert _rel_l2(got, ref) <= TOL_FP64
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(32, 8, 64), (7, 3, 5), (100, 1, 2), (512, 4, 512)])
+def test_tdma_cycl_many_bitwise_vs_reference_order(nx, ny, nz):
+    rng = np.random.default_rng(3 * nx + ny + nz)
+    a = rng.uniform(0.1, 1.0, (nz, ny, nx))
+    c = rng.uniform(0.1, 1.0, (nz, ny, nx))
+    b = -(a + c) - rng.uniform(0.5, 2.0, (nz, ny, nx))
+    d = rng.standard_normal((nz, ny, nx))
+    flat = [x.reshape(nz, -1) for x in (a, b, c, d)]
+    ref = O.tdma_cycl_many(*flat).reshape(d.shape)
+    plan = api.TDMAPlanMany(nx, ny, nz)
+    A, B, Cd, D = (api.DeviceArray.from_host(x) for x in (a, b, c, d))
+    plan.solve_cycle(A, B, Cd, D)
+    api.sync()
+    assert np.array_equal(D.download(), ref)
+    plan.destroy()

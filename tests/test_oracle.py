@@ -369,12 +369,66 @@ def test_c1_manufactured_accuracy():
     assert O.rel_residual(prob.rhs, P, prob.L, prob.pbc, az.dmx, az.dx) < 1e-12
 
 
-def test_solve_rejects_odd_and_periodic_z():
+def test_solve_rejects_odd_extents():
     f = np.zeros((4, 4, 5))
     ax, ay, az = make_axis(5, 1.0, False), make_axis(4, 1.0, False), make_axis(4, 1.0, False)
     with pytest.raises(ValueError):
         O.solve(f, (1, 1, 1), (False, False, False), ax.dmx, ay.dmx, az.dmx, az.dx)
-    ax = make_axis(4, 1.0, False)
-    f = np.zeros((4, 4, 4))
-    with pytest.raises(ValueError):
-        O.solve(f, (1, 1, 1), (False, False, True), ax.dmx, ay.dmx, az.dmx, az.dx)
+
+
+# ------------------------------------------------------ cyclic Thomas and periodic z
+
+
+def _random_cyclic_system(nrow, nsys, seed):
+    a, b, c, d = _random_dd_system(nrow, nsys, seed)
+    rng = np.random.default_rng(seed + 1)
+    a[0] = rng.uniform(-1, 1, nsys)   # corner couplings x_{n-1} -> row 0
+    c[-1] = rng.uniform(-1, 1, nsys)  # and x_0 -> row n-1
+    return a, b, c, d
+
+
+@pytest.mark.skipif(not O.ref_tdma_available(), reason="reference tdmas.cpp not built (no reference checkout)")
+@pytest.mark.parametrize("n", [3, 8, 33, 128])
+def test_cyclic_thomas_bitwise_vs_reference_tdmas(n):
+    """Square batches: the layout on which the reference C++ and Fortran
+    tdma_cycl_many agree (the C++ strides rows by the row count)."""
+    a, b, c, d = _random_cyclic_system(n, n, 100 + n)
+    assert np.array_equal(O.tdma_cycl_many(a, b, c, d), O.ref_tdma_cycl_many(a, b, c, d))
+
+
+@pytest.mark.parametrize("nrow,nsys", [(3, 1), (4, 7), (40, 6), (97, 3)])
+def test_cyclic_thomas_vs_dense_periodic_solve(nrow, nsys):
+    a, b, c, d = _random_cyclic_system(nrow, nsys, 7 * nrow + nsys)
+    x = O.tdma_cycl_many(a, b, c, d)
+    for s in range(nsys):
+        A = np.diag(b[:, s]) + np.diag(a[1:, s], -1) + np.diag(c[:-1, s], 1)
+        A[0, -1] += a[0, s]
+        A[-1, 0] += c[-1, s]
+        np.testing.assert_allclose(x[:, s], np.linalg.solve(A, d[:, s]), rtol=1e-11, atol=1e-12)
+
+
+def test_cyclic_thomas_with_zero_corners_is_plain_thomas():
+    a, b, c, d = _random_dd_system(30, 5, 11)
+    a[0] = 0.0
+    c[-1] = 0.0
+    np.testing.assert_allclose(O.tdma_cycl_many(a, b, c, d), O.tdma_many(a, b, c, d), rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("pbc", [(False, False), (True, True), (True, False), (False, True)])
+def test_periodic_z_follows_the_reference_rows(pbc):
+    """pbc3 as the reference computes it: the z rows keep am(1) = ap(n3m) = 0
+    (fft_3d_poisson.f90:1143-1146 zero them on the only rank whatever pbc3
+    says), so the cyclic solve has zero corners. The interior equals the wall
+    solve with the same (periodic) metrics; the z ghost planes are the wrapped
+    interior planes, taken after the x/y ghosts are set (:620-622)."""
+    n1, n2, n3 = 12, 8, 10
+    ax, ay = make_axis(n1, 1.0, pbc[0]), make_axis(n2, 1.3, pbc[1])
+    az = make_axis(n3, 2.0, True)
+    f = np.random.default_rng(3).standard_normal((n3, n2, n1))
+    f -= f.mean()
+    L = (1.0, 1.3, 2.0)
+    Pp = O.solve(f, L, (pbc[0], pbc[1], True), ax.dmx, ay.dmx, az.dmx, az.dx)
+    Pw = O.solve(f, L, (pbc[0], pbc[1], False), ax.dmx, ay.dmx, az.dmx, az.dx)
+    np.testing.assert_allclose(Pp[1:-1], Pw[1:-1], rtol=0, atol=1e-12 * np.abs(Pw).max())
+    assert np.array_equal(Pp[0], Pp[n3])
+    assert np.array_equal(Pp[n3 + 1], Pp[1])
