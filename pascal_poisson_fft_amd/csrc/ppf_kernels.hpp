@@ -41,6 +41,11 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
     else *p = v;
 }
 
+// Bytes of each W row a y-pass tile covers (its scalar columns x sizeof(T)).
+#ifndef PPF_YROW_BYTES
+#define PPF_YROW_BYTES 128
+#endif
+
 // Register budgets of the line-transform kernels, as minimum waves per SIMD
 // (occupancy is what hides the LDS and HBM latency of these kernels).
 #ifndef PPF_XWPE
@@ -407,7 +412,7 @@ struct YGeom {
     static constexpr int lds_bytes(int tx) {
         return ((CPLX ? tx / 2 : tx) * (2 * lds_slots(S) + 1) + 2 * (M + 2 * (M + 1))) * (int)sizeof(T);
     }
-    static constexpr int TX0 = 128 / (int)sizeof(T);
+    static constexpr int TX0 = PPF_YROW_BYTES / (int)sizeof(T);
     static constexpr int tx_fit(int tx) {
         return (tx > 2 && (lds_bytes(tx) > 98304 || (CPLX ? tx / 2 : tx) * TPL > kMaxNT)) ? tx_fit(tx / 2) : tx;
     }
@@ -1172,22 +1177,63 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
 // ---------------------------------------------------------------------------
 // FP64 residual r = f - L p on interior cells of the ghosted P (the operator
 // of residual_at), with per-workgroup sums of r^2, f^2 and p (deterministic
-// two-level reduction with k_reduce3).  Workgroups stride over the
-// n2m*n3m rows; threads over a row.
+// two-level reduction with k_reduce3).  A tile is 256 consecutive x cells of
+// one row j over KZ consecutive planes; each thread walks its column up the
+// planes keeping p(k-1), p(k), p(k+1) in registers, so each plane of P is
+// loaded once per column (its x/y neighbours come from the caches).
+// Workgroups stride over the tiles.
 // ---------------------------------------------------------------------------
+template <int KZ>
 __global__ void __launch_bounds__(256)
 k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m) {
     __shared__ double red[256];
+    const int nxc = (n1m + 255) / 256, nkb = (n3m + KZ - 1) / KZ;
+    const long ntiles = (long)nxc * n2m * nkb;
+    const long L1 = (long)n1m + 2, L2 = (long)n2m + 2, pl = L1 * L2;
+    const double *__restrict__ P = xr.P;
     double sr = 0.0, sf = 0.0, sx = 0.0;
-    const int nrows = n2m * n3m;
-    for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
-        const int k = row / n2m, j = row - k * n2m;
-        for (int i = threadIdx.x; i < n1m; i += blockDim.x) {
-            double fv, xv;
-            const double r = residual_at(f, xr, i, j, k, n1m, n2m, fv, xv);
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int xc = (int)(tile % nxc);
+        const long rest = tile / nxc;
+        const int j = (int)(rest % n2m), kb = (int)(rest / n2m);
+        const int i = xc * 256 + (int)threadIdx.x;
+        if (i >= n1m) continue;
+        // neighbour offsets: zero-flux walls reuse the centre, periodic axes wrap
+        long dxm = -1, dxp = 1, dym = -L1, dyp = L1;
+        if (xr.pbc1) {
+            if (i == 0) dxm = n1m - 1;
+            if (i == n1m - 1) dxp = -(long)(n1m - 1);
+        } else {
+            if (i == 0) dxm = 0;
+            if (i == n1m - 1) dxp = 0;
+        }
+        if (xr.pbc2) {
+            if (j == 0) dym = (long)(n2m - 1) * L1;
+            if (j == n2m - 1) dyp = -(long)(n2m - 1) * L1;
+        } else {
+            if (j == 0) dym = 0;
+            if (j == n2m - 1) dyp = 0;
+        }
+        const int k0 = kb * KZ;
+        const long pc0 = ((long)(k0 + 1) * L2 + (j + 1)) * L1 + (i + 1);
+        double zm = P[pc0 - pl], c = P[pc0];  // ghost plane below the first (finite, weight 0 at the wall)
+#pragma unroll
+        for (int kk = 0; kk < KZ; ++kk) {
+            const int k = k0 + kk;
+            if (k >= n3m) break;
+            const long pc = pc0 + (long)kk * pl;
+            const double zp = P[pc + pl];
+            const double xm = P[pc + dxm], xp = P[pc + dxp], ym = P[pc + dym], yp = P[pc + dyp];
+            const double fv = ld_stream(&f[((size_t)k * n2m + j) * n1m + i]);
+            const double am = xr.zam[k], ap = xr.zap[k];
+            const double lp = (xp - 2.0 * c + xm) * xr.ih1 + (yp - 2.0 * c + ym) * xr.ih2 + am * zm - (am + ap) * c +
+                              ap * zp;
+            const double r = fv - lp;
             sr += r * r;
             sf += fv * fv;
-            sx += xv;
+            sx += c;
+            zm = c;
+            c = zp;
         }
     }
     const double tr = block_sum<256>(sr, red);

@@ -684,8 +684,10 @@ int ppf_residual(ppf_plan *p, const double *f, const double *P, const double *dm
     const int n3m = p->d.n3m;
     hipLaunchKernelGGL(k_zcoef, dim3((n3m + 255) / 256), dim3(256), 0, s, dx3, dmx3, n3m, p->zcoef, p->zcoef + n3m);
     const XRes xr = make_xres(p, P);
-    const int g = std::min(p->npartial, p->d.n2m * n3m);
-    hipLaunchKernelGGL(k_residual, dim3(g), dim3(256), 0, s, f, xr, p->d.n1m, p->d.n2m, n3m);
+    constexpr int KZ = 8;
+    const long ntiles = (long)((p->d.n1m + 255) / 256) * p->d.n2m * ((n3m + KZ - 1) / KZ);
+    const int g = (int)std::min<long>(std::min<long>(p->npartial, 8L * num_cus()), ntiles);
+    hipLaunchKernelGGL(k_residual<KZ>, dim3(g), dim3(256), 0, s, f, xr, p->d.n1m, p->d.n2m, n3m);
     hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, s, p->partial, g, p->red);
     PPF_HIP(hipGetLastError());
     PPF_HIP(hipMemcpyAsync(p->hred, p->red, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
