@@ -97,6 +97,77 @@ def precision_leg(api, prob, dev, n, steps, fp64_ms, res_ms, fp64_res):
     }
 
 
+def batch_leg(api, W, reps: int = 5):
+    """96 independent problems of uneven shapes (mixed radix, periodic and
+    Neumann, uniform and tanh z): FP32+refinement to 1e-10 as one batch (one
+    hipGraph replay per batch solve, z shapes autotuned per problem shape)
+    against the same problems solved one at a time in plain FP64, each call
+    synchronised as the reference's solve is."""
+    probs = W.config4()
+    npts = sum(int(np.prod(p.rhs.shape)) for p in probs)
+    devs = [api.Problem3D(p.rhs, *p.ax) for p in probs]
+    out = {"workload": "96 problems, nx,ny in {32..256}, nz in [32,384], half periodic x,y, half Neumann, "
+                       "z uniform or tanh, smoothed N(0,1) RHS seed 4", "problems": len(probs), "points": npts}
+    try:
+        t0 = time.perf_counter()
+        spec = [dict(n1m=p.rhs.shape[2], n2m=p.rhs.shape[1], n3m=p.rhs.shape[0], L=p.L, pbc=p.pbc,
+                     precision=api.PPF_PREC_FP32_IR, tol=1e-10, max_sweeps=8) for p in probs]
+        b = api.Batch(list(zip(spec, devs)))
+        out["create_and_tune_s"] = round(time.perf_counter() - t0, 2)
+        st = b.solve(stats=True)  # eager: learns the sweeps, then captures the graph
+        b.solve()
+        api.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.solve()
+        api.sync()
+        tb = (time.perf_counter() - t0) / reps
+        st = b.solve(stats=True)
+        b.destroy()
+        res = [r for _, r in st]
+        out["fp32_ir_batch"] = {"ms_per_batch": round(tb * 1e3, 3), "solves_per_s": round(len(probs) / tb, 1),
+                                "mpts_per_s": round(npts / tb / 1e6, 1), "max_sweeps": max(s for s, _ in st),
+                                "max_rel_residual": max(res)}
+        # the same batch in plain FP64 (graph replay, no refinement)
+        spec64 = [dict(sp, precision=api.PPF_PREC_FP64) for sp in spec]
+        b = api.Batch(list(zip(spec64, devs)))
+        b.solve()
+        b.solve()
+        api.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.solve()
+        api.sync()
+        t64 = (time.perf_counter() - t0) / reps
+        st64 = b.solve(stats=True)
+        b.destroy()
+        out["fp64_batch"] = {"ms_per_batch": round(t64 * 1e3, 3), "solves_per_s": round(len(probs) / t64, 1),
+                             "mpts_per_s": round(npts / t64 / 1e6, 1),
+                             "max_rel_residual": max(r for _, r in st64)}
+        plans = [api.FFTPoisson(p.rhs.shape[2], p.rhs.shape[1], p.rhs.shape[0], p.L, p.pbc) for p in probs]
+        for pl, d in zip(plans, devs):
+            pl.solve(d.rhs, d.p, d.dmx1, d.dmx2, d.dmx3, d.dx3)
+        api.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for pl, d in zip(plans, devs):
+                pl.solve(d.rhs, d.p, d.dmx1, d.dmx2, d.dmx3, d.dx3)
+                api.sync()
+        t1 = (time.perf_counter() - t0) / reps
+        for pl in plans:
+            pl.clean()
+        out["fp64_one_at_a_time"] = {"ms_per_batch": round(t1 * 1e3, 3), "solves_per_s": round(len(probs) / t1, 1),
+                                     "mpts_per_s": round(npts / t1 / 1e6, 1)}
+        out["fp32_ir_batch_speedup_vs_fp64_one_at_a_time"] = round(t1 / tb, 3)
+        out["fp64_batch_speedup_vs_fp64_one_at_a_time"] = round(t1 / t64, 3)
+    except Exception as e:  # reported in the JSON line, not hidden
+        out["error"] = repr(e)
+    finally:
+        for d in devs:
+            d.free()
+    return out
+
+
 def load_traffic(kernel_key: str):
     """HBM bytes per launch of the dominant pass from committed rocprofv3 PMC
     summaries (profiles/traffic.json), or None."""
@@ -118,6 +189,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-ir", action="store_true", help="skip the FP32+refinement comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 96-problem batch leg")
     ap.add_argument("--cpu-nz", type=int, default=0, help="z extent of the CPU baseline slab (default n/4)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
@@ -208,6 +280,9 @@ def main(argv=None):
         "rel_residual": rel_res,
         "fp32_refine_vs_fp64": prec,
     }
+    if not a.no_batch:
+        print("# batch leg (96 problems)", file=sys.stderr, flush=True)
+        out["batch"] = batch_leg(api, W, reps=max(3, a.steps // 4))
     if not a.no_cpu:
         nz = a.cpu_nz or max(2, n // 4)
         print(f"# cpu baseline on {n}x{n}x{nz}", file=sys.stderr, flush=True)

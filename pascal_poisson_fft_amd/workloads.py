@@ -124,3 +124,23 @@ def smoothed_normal(shape, pbc, gamma: float, L=(2.0, 2.0, 2.0), seed: int = 2, 
 def config2(n: int = 512, seed: int = 2) -> Problem:
     """512^3 FP64, Neumann x and y, tanh z (stretch 1.5) with Neumann walls, L = 2."""
     return smoothed_normal((n, n, n), (False, False, False), 1.5, seed=seed)
+
+
+def config4(nprob: int = 96, seed: int = 4):
+    """Many independent problems of uneven shapes: nx, ny from
+    {32, 48, 64, 96, 128, 192, 256}, nz uniform in [32, 384] (even, as the
+    solver requires), the first half periodic in x and y, the second half
+    Neumann, z walls uniform (p = 1/2) or tanh-stretched (factor U[1, 2]),
+    L = 1, smoothed N(0,1) RHS with the dx3-weighted mean removed."""
+    rng = np.random.default_rng(seed)
+    sizes = np.array([32, 48, 64, 96, 128, 192, 256])
+    probs = []
+    for i in range(nprob):
+        n1, n2 = (int(v) for v in rng.choice(sizes, 2))
+        n3 = 2 * int(rng.integers(16, 193))
+        per = i < nprob // 2
+        uniform = bool(rng.random() < 0.5)
+        gamma = 0.0 if uniform else float(rng.uniform(1.0, 2.0))
+        probs.append(smoothed_normal((n1, n2, n3), (per, per, False), gamma, L=(1.0, 1.0, 1.0),
+                                     seed=seed * 1000 + i, uniform_z=uniform))
+    return probs
