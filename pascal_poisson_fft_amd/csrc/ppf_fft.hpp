@@ -7,10 +7,13 @@
 // place (Stockham ordering), so one LDS buffer per line suffices and the
 // result lands in natural order.
 //
-// Twiddles come from a table tw[m] = exp(-2 pi i m / M), m < M, in global
-// memory (read-only, L1/L2 resident).  Forward transforms use sign -1;
-// inverse transforms are obtained by the caller by conjugating on load and on
-// store (IDFT(x) = conj(DFT(conj(x)))).
+// Twiddles come from a per-stage table (stage_twiddles): the stage of span
+// LS and radix R owns entries [LS-1, LS-1 + (R-1) LS) holding
+// exp(-2 pi i j r / (LS R)) at (r-1) LS + j, so the lanes of a line (j
+// consecutive) read consecutive entries.  The spans telescope, so the table
+// has M-1 entries.  Forward transforms use sign -1; inverse transforms are
+// obtained by the caller by conjugating on load and on store
+// (IDFT(x) = conj(DFT(conj(x)))).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -153,7 +156,6 @@ template <typename T, int M, int TPL, int R, int LS>
 __device__ __forceinline__ void fft_stage(LLine<T> line, int t, const cx<T> *__restrict__ tw) {
     constexpr int S = M / R;                  // butterflies per line
     constexpr int NB = (S + TPL - 1) / TPL;   // butterflies per thread
-    constexpr int STEP = M / (LS * R);        // twiddle table stride
     cx<T> v[NB][R];
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
@@ -172,7 +174,7 @@ __device__ __forceinline__ void fft_stage(LLine<T> line, int t, const cx<T> *__r
             const int kk = b / LS;
             if (LS > 1) {
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], tw[j * r * STEP]);
+                for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], tw[(LS - 1) + (r - 1) * LS + j]);
             }
             Dft<T, R>::run(v[q]);
 #pragma unroll

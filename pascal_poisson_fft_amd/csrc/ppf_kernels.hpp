@@ -41,6 +41,17 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
     else *p = v;
 }
 
+// Divides the y passes' threads per line (smaller workgroups, more of them
+// per CU).
+#ifndef PPF_YTPL_DIV
+#define PPF_YTPL_DIV 1
+#endif
+
+// Rounds in which a y-pass workgroup transforms its tile's lines.
+#ifndef PPF_YROUNDS
+#define PPF_YROUNDS 1
+#endif
+
 // Bytes of each W row a y-pass tile covers (its scalar columns x sizeof(T)).
 #ifndef PPF_YROW_BYTES
 #define PPF_YROW_BYTES 128
@@ -405,7 +416,10 @@ template <typename T, int N, int MODE>
 struct YGeom {
     static constexpr bool CPLX = MODE == 1;
     static constexpr int M = CPLX ? N : N / 2;
-    static constexpr int TPL = (M / max_radix(M)) > 0 ? (M / max_radix(M)) : 1;
+    // threads per line: one radix-R butterfly each per stage, divided by
+    // PPF_YTPL_DIV (each thread then runs that many butterflies)
+    static constexpr int TPL0 = (M / max_radix(M)) > 0 ? (M / max_radix(M)) : 1;
+    static constexpr int TPL = TPL0 / PPF_YTPL_DIV > 0 ? TPL0 / PPF_YTPL_DIV : 1;
     static constexpr int S = CPLX ? M + 1 : (MODE == 2 ? M + 1 : M);
     // scalar columns per tile: 128-byte rows, LDS (lines + tables) within
     // 96 KiB, threads within kMaxNT
@@ -418,7 +432,10 @@ struct YGeom {
     }
     static constexpr int TX = tx_fit(TX0);
     static constexpr int NL = CPLX ? TX / 2 : TX;  // LDS lines per tile
-    static constexpr int NT = NL * TPL;
+    // the tile's lines are transformed in ROUNDS rounds by NL/ROUNDS line
+    // groups of TPL threads (fewer threads per workgroup, same LDS tile)
+    static constexpr int ROUNDS = (NL % PPF_YROUNDS == 0) ? PPF_YROUNDS : 1;
+    static constexpr int NT = NL / ROUNDS * TPL;
     using L = LdsLines<T, M, S, NL>;
 };
 
@@ -454,7 +471,6 @@ k_yfwd(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
     if (tile < ntiles) fetch(tile);
     L::load_tables(lds, tw, wk, qk, tid, NT);
     const cx<T> *tab = L::tables(lds);
-    const LLine<T> my = L::line(lds, l);
     for (; tile < ntiles; tile += gridDim.x) {
     const int c0 = (tile % nct) * TX;
     const size_t plane = (size_t)(tile / nct + 1) * Ly;
@@ -478,9 +494,13 @@ k_yfwd(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
     }
     if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
     __syncthreads();
-    fft_forward<T, M, TPL>(my, t, tab);
-    if (MODE == 0) dct_post<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
-    if (MODE == 2) r2c_post<T, N, TPL>(my, t, tab + M);
+#pragma unroll 1
+    for (int rd = 0; rd < Y::ROUNDS; ++rd) {
+        const LLine<T> my = L::line(lds, l + rd * (Y::NL / Y::ROUNDS));
+        fft_forward<T, M, TPL>(my, t, tab);
+        if (MODE == 0) dct_post<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
+        if (MODE == 2) r2c_post<T, N, TPL>(my, t, tab + M);
+    }
     constexpr int NR = MODE == 2 ? N + 2 : N;  // spectral rows written
     constexpr int R0 = MODE == 2 ? 0 : 1;
     for (int e = tid; e < TP * NR; e += NT) {
@@ -547,7 +567,6 @@ k_yinv(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
         bcb1 = (T)(e1 * e1 / (s1 * s1 - e1 * e1));
     }
     const cx<T> *tab = L::tables(lds);
-    const LLine<T> my = L::line(lds, l);
     for (; tile < ntiles; tile += gridDim.x) {
     const int c0 = (tile % nct) * TX;
     const size_t plane = (size_t)(tile / nct) * Ly;
@@ -568,9 +587,13 @@ k_yinv(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
     }
     if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
     __syncthreads();
-    if (MODE == 0) idct_pre<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
-    if (MODE == 2) c2r_pre<T, N, TPL>(my, t, tab + M);
-    fft_forward<T, M, TPL>(my, t, tab);
+#pragma unroll 1
+    for (int rd = 0; rd < Y::ROUNDS; ++rd) {
+        const LLine<T> my = L::line(lds, l + rd * (Y::NL / Y::ROUNDS));
+        if (MODE == 0) idct_pre<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
+        if (MODE == 2) c2r_pre<T, N, TPL>(my, t, tab + M);
+        fft_forward<T, M, TPL>(my, t, tab);
+    }
     for (int e = tid; e < TP * (N + 2); e += NT) {
         const int rp = e / TP, c2 = e - rp * TP;
         const int col = c0 + 2 * c2;

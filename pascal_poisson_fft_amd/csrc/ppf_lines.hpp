@@ -19,7 +19,7 @@
 // costs one half-length complex FFT.
 //
 // Table layout per axis (device, type cx<T>):
-//   tw[m]   = exp(-2 pi i m / M), m < M        (FFT)
+//   tw      per-stage FFT twiddles, M-1 entries (ppf_fft.hpp)
 //   wk[k]   = exp(-2 pi i k / N), k <= M       (real FFT split)
 //   qk[k]   = exp(-i pi k / (2N)), k <= M      (DCT twiddle)
 #pragma once

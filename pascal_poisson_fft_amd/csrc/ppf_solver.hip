@@ -93,10 +93,16 @@ static int dev_alloc(ppf_plan *p, T **ptr, size_t count) {
 template <typename T>
 static int upload_tables(ppf_plan *p, AxisTables<T> &tb, int N, int kind) {
     const int M = (kind == AX_C2C) ? N : N / 2;
-    std::vector<cx<T>> tw(M), wk(M + 1), qk(M + 1);
-    for (int m = 0; m < M; ++m) {
-        long double a = -2.0L * (long double)kPi * (long double)m / (long double)M;
-        tw[m] = {(T)cosl(a), (T)sinl(a)};
+    std::vector<cx<T>> tw(M, cx<T>{T(1), T(0)}), wk(M + 1), qk(M + 1);
+    // per-stage FFT twiddles: stage of span LS, radix R at LS-1 + (r-1) LS + j
+    for (int LS = 1; LS < M;) {
+        const int R = pick_radix(M / LS);
+        for (int r = 1; r < R; ++r)
+            for (int j = 0; j < LS; ++j) {
+                long double a = -2.0L * (long double)kPi * (long double)(j * r) / (long double)(LS * R);
+                tw[(size_t)(LS - 1) + (size_t)(r - 1) * LS + j] = {(T)cosl(a), (T)sinl(a)};
+            }
+        LS *= R;
     }
     for (int k = 0; k <= M; ++k) {
         long double a = -2.0L * (long double)kPi * (long double)k / (long double)N;
