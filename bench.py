@@ -168,6 +168,95 @@ def batch_leg(api, W, reps: int = 5):
     return out
 
 
+def stream_leg(api, W, nsteps: int = 200, n=(1024, 1024, 768)):
+    """Projection-step stream: 1024x1024x768, periodic x, Neumann y, tanh z
+    (stretch 1.8); nsteps successive right-hand sides, each the divergence of
+    a fresh random staggered velocity field generated on the GPU
+    (ppf_synth_divergence_rhs, seed 5+t), solved (a) in plain FP64 and (b) in
+    FP32+refinement to 1e-10 replayed from a hipGraph (a one-problem batch
+    whose sweep count is learned on the first step).  Per-solve wall times
+    (solve + synchronise) give p50/p99."""
+    from pascal_poisson_fft_amd.grid import make_axis
+    n1, n2, n3 = n
+    L = (2.0, 2.0, 2.0)
+    ax, ay, az = make_axis(n1, L[0], True), make_axis(n2, L[1], False), make_axis(n3, L[2], False, uniform=False,
+                                                                                    gamma=1.8)
+    npts = n1 * n2 * n3
+    out = {"workload": f"{n1}x{n2}x{n3} periodic x, Neumann y, tanh z 1.8, {nsteps} RHS = divergence of random "
+                       "staggered velocity fields (16 separable modes, k^-5/3 amplitudes, face noise 0.1), seed 5+t",
+           "steps": nsteps, "points": npts}
+    rhs = api.DeviceArray((n3, n2, n1))
+    P = api.DeviceArray((n3 + 2, n2 + 2, n1 + 2))
+    P.fill_zero()
+    dmx1, dmx2 = api.DeviceArray.from_host(ax.dmx), api.DeviceArray.from_host(ay.dmx)
+    dmx3, dx3 = api.DeviceArray.from_host(az.dmx), api.DeviceArray.from_host(az.dx)
+    tabs = None
+    bufs = [rhs, P, dmx1, dmx2, dmx3, dx3]
+
+    class _Dev:  # the buffer bundle Batch expects
+        pass
+    dev = _Dev()
+    dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3 = rhs, P, dmx1, dmx2, dmx3, dx3
+
+    def gen(t):
+        nonlocal tabs
+        T, stride = W.projection_tables(ax, ay, az, nmodes=16, seed=5 + t)
+        if tabs is None:
+            tabs = api.DeviceArray(T.shape)
+            bufs.append(tabs)
+        tabs.upload(np.ascontiguousarray(T))
+        api.synth_divergence_rhs(rhs, n1, n2, n3, L[0], L[1], dx3, tabs, 16, stride, 5 + t, 0.1)
+
+    try:
+        plan = api.FFTPoisson(n1, n2, n3, L, (True, False, False))
+        args = (rhs, P, dmx1, dmx2, dmx3, dx3)
+        gen(0)
+        plan.solve(*args)
+        api.sync()
+        ts, res = [], []
+        for t in range(nsteps):
+            gen(t)
+            api.sync()
+            t0 = time.perf_counter()
+            plan.solve(*args)
+            api.sync()
+            ts.append(time.perf_counter() - t0)
+            if t % 50 == 0:
+                res.append(plan.residual(rhs, P, dmx3, dx3))
+        plan.clean()
+        ts = np.array(ts) * 1e3
+        out["fp64"] = {"p50_ms": round(float(np.percentile(ts, 50)), 3), "p99_ms": round(float(np.percentile(ts, 99)), 3),
+                       "roofline_frac_p50": round(80.0 * npts / (np.percentile(ts, 50) * 1e-3) / 8e12, 4),
+                       "max_rel_residual": max(res)}
+        spec = dict(n1m=n1, n2m=n2, n3m=n3, L=L, pbc=(True, False, False), precision=api.PPF_PREC_FP32_IR,
+                    tol=1e-10, max_sweeps=8)
+        b = api.Batch([(spec, dev)])
+        gen(0)
+        b.solve(stats=True)  # eager: learns the sweeps, captures the graph
+        ts, res = [], []
+        for t in range(nsteps):
+            gen(t)
+            api.sync()
+            t0 = time.perf_counter()
+            b.solve()
+            api.sync()
+            ts.append(time.perf_counter() - t0)
+            if t % 50 == 0:
+                res.append(b.solve(stats=True)[0][1])
+        sw = b.solve(stats=True)[0][0]
+        b.destroy()
+        ts = np.array(ts) * 1e3
+        out["fp32_ir_graph"] = {"p50_ms": round(float(np.percentile(ts, 50)), 3),
+                                "p99_ms": round(float(np.percentile(ts, 99)), 3), "sweeps": int(sw),
+                                "max_rel_residual": max(res)}
+    except Exception as e:  # reported in the JSON line, not hidden
+        out["error"] = repr(e)
+    finally:
+        for a in bufs:
+            a.free()
+    return out
+
+
 def load_traffic(kernel_key: str):
     """HBM bytes per launch of the dominant pass from committed rocprofv3 PMC
     summaries (profiles/traffic.json), or None."""
@@ -190,6 +279,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-ir", action="store_true", help="skip the FP32+refinement comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the 96-problem batch leg")
+    ap.add_argument("--stream-steps", type=int, default=200,
+                    help="right-hand sides of the 1024x1024x768 projection stream (0 skips it)")
     ap.add_argument("--cpu-nz", type=int, default=0, help="z extent of the CPU baseline slab (default n/4)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
@@ -283,6 +374,9 @@ def main(argv=None):
     if not a.no_batch:
         print("# batch leg (96 problems)", file=sys.stderr, flush=True)
         out["batch"] = batch_leg(api, W, reps=max(3, a.steps // 4))
+    if a.stream_steps > 0:
+        print(f"# projection stream ({a.stream_steps} RHS)", file=sys.stderr, flush=True)
+        out["projection_stream"] = stream_leg(api, W, a.stream_steps)
     if not a.no_cpu:
         nz = a.cpu_nz or max(2, n // 4)
         print(f"# cpu baseline on {n}x{n}x{nz}", file=sys.stderr, flush=True)

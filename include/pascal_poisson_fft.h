@@ -207,6 +207,20 @@ int ppf_set_device(int device);
  * and the known byte count calibrates rocprofv3's HBM byte counters. */
 int ppf_copy_timed(double *dst, const double *src, size_t count, int reps, void *stream, float *ms);
 
+/* Synthetic projection-step right-hand side (measurement workload, no
+ * counterpart in the reference): f (n1m, n2m, n3m) = the discrete divergence
+ * of a staggered velocity field -- u on x-faces (periodic in x), v on y-faces
+ * and w on z-faces (both zero on the walls) -- each a sum of nmodes separable
+ * modes from `tables` (device, 9 blocks of nmodes x stride doubles: u's
+ * x-face / y-centre / z-centre factors, v's x-centre / y-face / z-centre, w's
+ * x-centre / y-centre / z-face; stride >= max(n1m, n2m, n3m) + 1) plus
+ * seeded uniform face noise of amplitude `noise`.  x and y spacings are
+ * L1/n1m and L2/n2m; dx3 (device, n3m+2) gives the z cell widths.  Zero
+ * wall fluxes make sum f dx3 = 0 (compatible by construction). */
+int ppf_synth_divergence_rhs(double *f, int n1m, int n2m, int n3m, double L1, double L2, const double *dx3,
+                             const double *tables, int nmodes, int stride, unsigned long long seed, double noise,
+                             void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -98,6 +98,7 @@ def lib() -> C.CDLL:
     L.ppf_batch_create.argtypes = [I, C.POINTER(Desc), C.POINTER(BatchIO), I, C.POINTER(vp)]
     L.ppf_batch_solve.argtypes = [vp, vp, C.POINTER(Stats)]
     L.ppf_batch_destroy.argtypes = [vp]
+    L.ppf_synth_divergence_rhs.argtypes = [vp, I, I, I, D, D, vp, vp, I, I, C.c_ulonglong, D, vp]
     _lib = L
     return L
 
@@ -119,6 +120,16 @@ def device_count() -> int:
 
 def sync(stream=None) -> None:
     _check(lib().ppf_sync(stream), "ppf_sync")
+
+
+def synth_divergence_rhs(f: "DeviceArray", n1m: int, n2m: int, n3m: int, L1: float, L2: float,
+                         dx3: "DeviceArray", tables: "DeviceArray", nmodes: int, stride: int, seed: int,
+                         noise: float, stream=None) -> None:
+    """Projection-step RHS: divergence of a synthetic staggered velocity field
+    (ppf_synth_divergence_rhs)."""
+    _check(lib().ppf_synth_divergence_rhs(f.ptr, n1m, n2m, n3m, float(L1), float(L2), dx3.ptr,
+                                          tables.ptr if tables is not None else None, int(nmodes), int(stride),
+                                          int(seed), float(noise), stream), "ppf_synth_divergence_rhs")
 
 
 def copy_bandwidth(count: int = 1 << 28, reps: int = 10) -> float:

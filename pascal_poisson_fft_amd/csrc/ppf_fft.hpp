@@ -20,8 +20,13 @@
 
 namespace ppf {
 
+// complex value; PPF_CX_ALIGN aligns it to its size (16 B in FP64) so LDS
+// and global accesses can be single 128-bit operations
+#ifndef PPF_CX_ALIGN
+#define PPF_CX_ALIGN 0
+#endif
 template <typename T>
-struct cx {
+struct alignas(PPF_CX_ALIGN ? 2 * sizeof(T) : alignof(T)) cx {
     T x, y;
 };
 

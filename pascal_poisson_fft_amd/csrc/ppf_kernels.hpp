@@ -47,6 +47,12 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
 #define PPF_YTPL_DIV 1
 #endif
 
+// Scalars of padding between consecutive LDS lines (odd: lines start on
+// different banks; 2: lines stay 16-byte aligned).
+#ifndef PPF_LINE_PAD
+#define PPF_LINE_PAD 2
+#endif
+
 // Rounds in which a y-pass workgroup transforms its tile's lines.
 #ifndef PPF_YROUNDS
 #define PPF_YROUNDS 1
@@ -73,7 +79,7 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
 // together with the data so no FFT stage waits on a global round trip.
 template <typename T, int M, int S, int NL>
 struct LdsLines {
-    static constexpr int LSD = 2 * lds_slots(S) + 1;  // S complex values, padded
+    static constexpr int LSD = 2 * lds_slots(S) + PPF_LINE_PAD;  // S complex values + pad scalars
     static constexpr int TABN = M + 2 * (M + 1);
     static constexpr int SIZE = NL * LSD + 2 * TABN;  // scalars of type T
     __device__ __forceinline__ static LLine<T> line(T *lds, int l) {
@@ -193,7 +199,7 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
     constexpr int EPT = (LPB * H + NT - 1) / NT;  // pair loads per thread per tile
     using D2 = double __attribute__((ext_vector_type(2)));
     using V2 = T __attribute__((ext_vector_type(2)));
-    __shared__ T lds[L::SIZE];
+    __shared__ __align__(16) T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int ntiles = (nlines + LPB - 1) / LPB;
     const float rn2m = 1.0f / (float)n2m;
@@ -303,7 +309,7 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
     constexpr int EPTP = (LPB * HP + NT - 1) / NT;  // P pairs stored per thread per tile
     using V2 = T __attribute__((ext_vector_type(2)));
     using D2 = double __attribute__((ext_vector_type(2)));
-    __shared__ T lds[L::SIZE];
+    __shared__ __align__(16) T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int ntiles = (nlines + LPB - 1) / LPB;
     V2 pre[EPT];
@@ -421,21 +427,23 @@ struct YGeom {
     static constexpr int TPL0 = (M / max_radix(M)) > 0 ? (M / max_radix(M)) : 1;
     static constexpr int TPL = TPL0 / PPF_YTPL_DIV > 0 ? TPL0 / PPF_YTPL_DIV : 1;
     static constexpr int S = CPLX ? M + 1 : (MODE == 2 ? M + 1 : M);
-    // scalar columns per tile: 128-byte rows, LDS (lines + tables) within
-    // 96 KiB, threads within kMaxNT
+    // scalar columns per tile: PPF_YROW_BYTES-wide rows (halved only if the
+    // LDS tile -- lines + tables -- would exceed the CU's 160 KiB); lines
+    // beyond kMaxNT / TPL are transformed in further rounds
     static constexpr int lds_bytes(int tx) {
-        return ((CPLX ? tx / 2 : tx) * (2 * lds_slots(S) + 1) + 2 * (M + 2 * (M + 1))) * (int)sizeof(T);
+        return ((CPLX ? tx / 2 : tx) * (2 * lds_slots(S) + PPF_LINE_PAD) + 2 * (M + 2 * (M + 1))) * (int)sizeof(T);
     }
     static constexpr int TX0 = PPF_YROW_BYTES / (int)sizeof(T);
-    static constexpr int tx_fit(int tx) {
-        return (tx > 2 && (lds_bytes(tx) > 98304 || (CPLX ? tx / 2 : tx) * TPL > kMaxNT)) ? tx_fit(tx / 2) : tx;
-    }
+    static constexpr int tx_fit(int tx) { return (tx > 2 && lds_bytes(tx) > 160 * 1024 - 1024) ? tx_fit(tx / 2) : tx; }
     static constexpr int TX = tx_fit(TX0);
     static constexpr int NL = CPLX ? TX / 2 : TX;  // LDS lines per tile
-    // the tile's lines are transformed in ROUNDS rounds by NL/ROUNDS line
-    // groups of TPL threads (fewer threads per workgroup, same LDS tile)
-    static constexpr int ROUNDS = (NL % PPF_YROUNDS == 0) ? PPF_YROUNDS : 1;
+    static constexpr int rounds_for(int r) {
+        return (NL / r) * TPL <= kMaxNT || NL % (2 * r) != 0 ? r : rounds_for(2 * r);
+    }
+    static constexpr int ROUNDS = rounds_for(PPF_YROUNDS);
     static constexpr int NT = NL / ROUNDS * TPL;
+    static_assert(NT <= kMaxNT, "y-pass workgroup exceeds kMaxNT threads");
+    static_assert(lds_bytes(TX) <= 160 * 1024, "y-pass tile exceeds the CU's LDS");
     using L = LdsLines<T, M, S, NL>;
 };
 
@@ -449,7 +457,7 @@ k_yfwd(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
     constexpr int TP = TX / 2;  // column pairs: global accesses are 2 adjacent scalars
     constexpr int EPT = (TP * N + NT - 1) / NT;
     using V2 = T __attribute__((ext_vector_type(2)));
-    __shared__ T lds[L::SIZE];
+    __shared__ __align__(16) T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int nct = (Lx + TX - 1) / TX;
     const int ntiles = nct * nplanes;
@@ -533,7 +541,7 @@ k_yinv(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
     constexpr int TP = TX / 2;  // column pairs: global accesses are 2 adjacent scalars
     constexpr int EPT = (TP * NR + NT - 1) / NT;
     using V2 = T __attribute__((ext_vector_type(2)));
-    __shared__ T lds[L::SIZE];
+    __shared__ __align__(16) T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int nct = (Lx + TX - 1) / TX;
     const int ntiles = nct * nplanes;  // every plane incl. the z ghost planes
@@ -743,13 +751,19 @@ k_zthomas(T *__restrict__ W, T *__restrict__ cs, ZArgs z, const double *__restri
 // and C).  Rows past n3m are identity rows.  Coefficients, pin, ghost planes
 // and mean as in k_zthomas.
 // ---------------------------------------------------------------------------
+#ifndef PPF_RCP_NEWTON
+#define PPF_RCP_NEWTON 2
+#endif
 __device__ __forceinline__ double frcp(double x) {
-    // reciprocal: hardware estimate + two Newton steps (about 1 ulp)
+    // reciprocal: hardware estimate refined by Newton steps (each doubles the
+    // correct bits; two give about 1 ulp)
     double r = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-x, r, 1.0);
-    return __builtin_fma(r, e, r);
+#pragma unroll
+    for (int i = 0; i < PPF_RCP_NEWTON; ++i) {
+        const double e = __builtin_fma(-x, r, 1.0);
+        r = __builtin_fma(r, e, r);
+    }
+    return r;
 }
 
 // z row coefficients for one solve: zam[k], zap[k] (k < n3m) as in

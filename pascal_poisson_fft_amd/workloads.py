@@ -144,3 +144,39 @@ def config4(nprob: int = 96, seed: int = 4):
         probs.append(smoothed_normal((n1, n2, n3), (per, per, False), gamma, L=(1.0, 1.0, 1.0),
                                      seed=seed * 1000 + i, uniform_z=uniform))
     return probs
+
+
+def projection_tables(ax: Axis, ay: Axis, az: Axis, nmodes: int = 16, seed: int = 5, kmax: int = 24):
+    """1-D mode factors of a random staggered velocity field for
+    ppf_synth_divergence_rhs: nmodes separable modes with integer wavenumbers
+    in [1, kmax] per axis, N(0,1) amplitudes scaled by |k|^(-5/6) (a k^-5/3
+    energy spectrum), x periodic (sin/cos of 2 pi k x / L1), y and z walls
+    (the wall-normal component is sin(pi k s / L), zero on both walls).
+    Returns (tables[9, nmodes, stride], stride)."""
+    rng = np.random.default_rng(seed)
+    n1, n2, n3 = ax.nm, ay.nm, az.nm
+    stride = max(n1, n2, n3) + 1
+    T = np.zeros((9, nmodes, stride))
+    xf = np.arange(n1) * (ax.L / n1)                      # x-faces (periodic)
+    xc = (np.arange(n1) + 0.5) * (ax.L / n1)
+    yf = np.arange(n2 + 1) * (ay.L / n2)                  # y-faces incl. both walls
+    yc = (np.arange(n2) + 0.5) * (ay.L / n2)
+    zf = az.x[1:n3 + 2] - az.x[1]                         # z-faces incl. both walls
+    zc = 0.5 * (az.x[1:n3 + 1] + az.x[2:n3 + 2]) - az.x[1]
+    Lz = zf[-1]
+    for m in range(nmodes):
+        kx, ky, kz = rng.integers(1, kmax + 1, 3)
+        ph = rng.uniform(0, 2 * np.pi)
+        a = rng.standard_normal(3) * float(np.sqrt(kx * kx + ky * ky + kz * kz)) ** (-5.0 / 6.0)
+        T[0, m, :n1] = a[0] * np.sin(2 * np.pi * kx * xf / ax.L + ph)
+        T[1, m, :n2] = np.cos(np.pi * ky * yc / ay.L)
+        T[2, m, :n3] = np.cos(np.pi * kz * zc / Lz)
+        T[3, m, :n1] = a[1] * np.cos(2 * np.pi * kx * xc / ax.L + ph)
+        T[4, m, :n2 + 1] = np.sin(np.pi * ky * yf / ay.L)
+        T[4, m, 0] = T[4, m, n2] = 0.0
+        T[5, m, :n3] = np.cos(np.pi * kz * zc / Lz)
+        T[6, m, :n1] = a[2] * np.cos(2 * np.pi * kx * xc / ax.L + ph)
+        T[7, m, :n2] = np.cos(np.pi * ky * yc / ay.L)
+        T[8, m, :n3 + 1] = np.sin(np.pi * kz * zf / Lz)
+        T[8, m, 0] = T[8, m, n3] = 0.0
+    return T, stride
