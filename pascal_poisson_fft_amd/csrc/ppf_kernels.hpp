@@ -1,7 +1,7 @@
 // ppf_kernels.hpp -- the five passes of the solve for gfx950.
 //
 // Workspace W (type T) holds one spectral field laid out x fastest:
-//   W[(kp * Ly + jp) * Lx + i],  Lx = n1m (Neumann x) or n1m+2 (periodic x,
+//   W[(kp * Ly + jp) * Lxs + i],  i < Lx = n1m (Neumann x) or n1m+2 (periodic x,
 //   half spectrum as interleaved re/im), Ly = n2m+2, Lz = n3m+2.
 // The forward passes use rows jp = 1..n2m and planes kp = 1..n3m; the z pass
 // adds the two spectral ghost planes kp = 0, n3m+1; the y-inverse fills the
@@ -20,6 +20,12 @@
 namespace ppf {
 
 constexpr int kMaxNT = 512;
+
+// Row stride of the spectral workspace: the Lx scalars of a row rounded up to
+// a multiple of 128 bytes, so every column tile of the y and z passes starts
+// on a cache-line boundary (a periodic-x row of n1m+2 scalars otherwise
+// straddles two lines in every tile segment).
+__host__ __device__ constexpr int w_stride(int lx, int tbytes) { return (lx * tbytes + 127) / 128 * 128 / tbytes; }
 
 // Streaming global accesses of the pass kernels: every element of a pass is
 // read once and written once, so PPF_NT_LOAD / PPF_NT_STORE (default on)
@@ -195,6 +201,7 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
     using L = typename X::L;
     constexpr int M = G::M, TPL = X::TPL, LPB = X::LPB, NT = X::NT;
     constexpr int LX = PER ? N + 2 : N;
+    constexpr int LXS = w_stride(LX, (int)sizeof(T));  // W row stride
     constexpr int H = N / 2, HX = LX / 2;          // sample pairs per line / per W row
     constexpr int EPT = (LPB * H + NT - 1) / NT;  // pair loads per thread per tile
     using D2 = double __attribute__((ext_vector_type(2)));
@@ -267,7 +274,7 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
                 // W row of prhs line (j,k): (k+1)*Ly + j + 1 = line + n2m + 3 + 2k
                 const int k = fdiv_small(line, n2m, rn2m);
                 const cx<T> z = L::line(lds, li)[h];
-                st_stream(reinterpret_cast<V2 *>(&W[((size_t)line + (size_t)(Ly + 1) + 2 * (size_t)k) * LX + 2 * h]),
+                st_stream(reinterpret_cast<V2 *>(&W[((size_t)line + (size_t)(Ly + 1) + 2 * (size_t)k) * LXS + 2 * h]),
                           V2{z.x, z.y});
             }
         }
@@ -303,6 +310,7 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
     using L = typename X::L;
     constexpr int M = G::M, TPL = X::TPL, LPB = X::LPB, NT = X::NT;
     constexpr int LX = PER ? N + 2 : N;
+    constexpr int LXS = w_stride(LX, (int)sizeof(T));  // W row stride
     constexpr int LP = N + 2;
     constexpr int HX = LX / 2, HP = LP / 2;  // scalar pairs per W row / per P row
     constexpr int EPT = (LPB * HX + NT - 1) / NT;
@@ -320,9 +328,10 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
         for (int q = 0; q < EPT; ++q) {
             int e = tid + q * NT;
             e = e < LPB * HX ? e : LPB * HX - 1;
-            size_t idx = (size_t)tile * LPB * HX + e;
-            const size_t last = (size_t)nlines * HX - 1;
-            pre[q] = ld_stream(reinterpret_cast<const V2 *>(W) + (idx < last ? idx : last));
+            const int li = e / HX, h = e - li * HX;
+            int row = tile * LPB + li;
+            row = row < nlines ? row : nlines - 1;
+            pre[q] = ld_stream(reinterpret_cast<const V2 *>(W + (size_t)row * LXS) + h);
         }
         if constexpr (ACC) {
 #pragma unroll
@@ -449,7 +458,7 @@ struct YGeom {
 
 template <typename T, int N, int MODE>
 __global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(PPF_YWPE)))
-k_yfwd(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
+k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
        const cx<T> *__restrict__ qk) {
     using Y = YGeom<T, N, MODE>;
     using L = typename Y::L;
@@ -472,7 +481,7 @@ k_yfwd(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
             const int r = e / TP, c2 = e - r * TP;
             int col = ct * TX + 2 * c2;
             col = col < Lx ? col : Lx - 2;  // Lx is even
-            pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[(plane + 1 + r) * Lx + col]));
+            pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[(plane + 1 + r) * Lxs + col]));
         }
     };
     int tile = blockIdx.x;
@@ -524,14 +533,14 @@ k_yfwd(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
             v.x = L::real(lds, 2 * c2)[r];
             v.y = L::real(lds, 2 * c2 + 1)[r];
         }
-        st_stream(reinterpret_cast<V2 *>(&W[(plane + R0 + r) * Lx + col]), v);
+        st_stream(reinterpret_cast<V2 *>(&W[(plane + R0 + r) * Lxs + col]), v);
     }
     }
 }
 
 template <typename T, int N, int MODE>
 __global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(PPF_YWPE)))
-k_yinv(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
+k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
        const cx<T> *__restrict__ qk, const double *__restrict__ dmx2) {
     using Y = YGeom<T, N, MODE>;
     using L = typename Y::L;
@@ -556,7 +565,7 @@ k_yinv(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
             const int r = e / TP, c2 = e - r * TP;
             int col = ct * TX + 2 * c2;
             col = col < Lx ? col : Lx - 2;  // Lx is even
-            pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[(plane + R0 + r) * Lx + col]));
+            pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[(plane + R0 + r) * Lxs + col]));
         }
     };
     int tile = blockIdx.x;
@@ -628,7 +637,7 @@ k_yinv(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
             out = MODE == 0 ? bca1 * sample(N - 1) - bcb1 * sample(N - 2) : sample(0);
         else
             out = sample(rp - 1);
-        st_stream(reinterpret_cast<V2 *>(&W[(plane + rp) * Lx + col]), out);
+        st_stream(reinterpret_cast<V2 *>(&W[(plane + rp) * Lxs + col]), out);
     }
     }
 }
@@ -644,7 +653,7 @@ k_yinv(T *__restrict__ W, int Lx, int Ly, int nplanes, const cx<T> *__restrict__
 // the (0,0) real column stores the solution mean for pass 5 (:628-664).
 // ---------------------------------------------------------------------------
 struct ZArgs {
-    int Lx, Ly, n3m, jlo, njr;
+    int Lx, Lxs, Ly, n3m, jlo, njr;   // Lx columns per row, Lxs the row stride
     const double *lamx, *lamy;        // per scalar column / per row
     const unsigned char *zx, *zy;     // zero-mode flags
     int mean_kx, mean_jp;             // real (0,0) column
@@ -682,8 +691,8 @@ k_zthomas(T *__restrict__ W, T *__restrict__ cs, ZArgs z, const double *__restri
     if (col >= (long)z.Lx * z.njr) return;
     const int kx = (int)(col % z.Lx);
     const int jp = z.jlo + (int)(col / z.Lx);
-    const size_t pstride = (size_t)z.Ly * z.Lx;
-    T *w = W + (size_t)jp * z.Lx + kx;       // plane 0 of this column
+    const size_t pstride = (size_t)z.Ly * z.Lxs;
+    T *w = W + (size_t)jp * z.Lxs + kx;      // plane 0 of this column
     T *c = cs + (size_t)(col);               // scratch, stride Lx*njr
     const size_t cstride = (size_t)z.Lx * z.njr;
     const double lx = z.lamx[kx], ly = z.lamy[jp];
@@ -808,8 +817,8 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     const int jp = z.jlo + blockIdx.x / tiles;
     const int kx = (blockIdx.x % tiles) * S + s;
     const bool active = kx < z.Lx;
-    const size_t pstride = (size_t)z.Ly * z.Lx;
-    T *w = W + (size_t)jp * z.Lx + (active ? kx : 0);
+    const size_t pstride = (size_t)z.Ly * z.Lxs;
+    T *w = W + (size_t)jp * z.Lxs + (active ? kx : 0);
     const double lx = active ? z.lamx[kx] : 0.0;
     const double ly = z.lamy[jp];
     const bool pin = active && z.zx[kx] && z.zy[jp];
@@ -1042,7 +1051,7 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     for (int q = 0; q < 4; ++q) bc[q] = zap[n3m + q];
     const int tiles_x = (z.Lx + S - 1) / S;
     const int ntiles = tiles_x * z.njr;
-    const size_t pstride = (size_t)z.Ly * z.Lx;
+    const size_t pstride = (size_t)z.Ly * z.Lxs;
     const int g0 = p * K;
     const int glast = n3m - 1;
     // Prefetch loads go to clamped, always valid addresses and nothing consumes
@@ -1055,7 +1064,7 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
         const int jp = z.jlo + tile / tiles_x;
         int kx = (tile % tiles_x) * S + s;
         kx = kx < z.Lx ? kx : z.Lx - 1;
-        const T *w = W + (size_t)jp * z.Lx + kx;
+        const T *w = W + (size_t)jp * z.Lxs + kx;
 #pragma unroll
         for (int l = 0; l < K; ++l) {
             const int g = g0 + l;
@@ -1073,7 +1082,7 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
         const int jp = z.jlo + tile / tiles_x;
         const int kx = (tile % tiles_x) * S + s;
         const bool active = kx < z.Lx;
-        T *w = W + (size_t)jp * z.Lx + (active ? kx : 0);
+        T *w = W + (size_t)jp * z.Lxs + (active ? kx : 0);
         const double lx = active ? pre_lx : 0.0;
         const double ly = pre_ly;
         const bool pin = active && pre_zx && pre_zy;

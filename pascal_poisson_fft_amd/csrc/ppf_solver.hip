@@ -49,6 +49,7 @@ constexpr int kMaxSlabs = 16;
 struct ppf_plan {
     ppf_desc d;
     int Lx, Ly, Lz;        // workspace extents (scalars)
+    int Lxs;               // workspace row stride (Lx rounded up to 128 bytes)
     int xkind, ykind;      // AxisKind
     int ymode;             // y pass MODE (0 DCT, 1 C2C, 2 R2C)
     int zjlo, znjr;        // rows solved by the z pass
@@ -218,11 +219,11 @@ static void launch_y_mode(const ppf_plan *p, T *W, const AxisTables<T> &tb, bool
     const int nb = (p->Lx + Y::TX - 1) / Y::TX;
     if (!inverse) {
         const unsigned g = persistent_grid(k_yfwd<T, N, MODE>, Y::NT, nb * nplanes);
-        hipLaunchKernelGGL((k_yfwd<T, N, MODE>), dim3(g), dim3(Y::NT), 0, s, W, p->Lx, p->Ly, nplanes, tb.tw, tb.wk,
+        hipLaunchKernelGGL((k_yfwd<T, N, MODE>), dim3(g), dim3(Y::NT), 0, s, W, p->Lx, p->Lxs, p->Ly, nplanes, tb.tw, tb.wk,
                            tb.qk);
     } else {
         const unsigned g = persistent_grid(k_yinv<T, N, MODE>, Y::NT, nb * nplanes);
-        hipLaunchKernelGGL((k_yinv<T, N, MODE>), dim3(g), dim3(Y::NT), 0, s, W, p->Lx, p->Ly, nplanes, tb.tw, tb.wk,
+        hipLaunchKernelGGL((k_yinv<T, N, MODE>), dim3(g), dim3(Y::NT), 0, s, W, p->Lx, p->Lxs, p->Ly, nplanes, tb.tw, tb.wk,
                            tb.qk, dmx2);
     }
 }
@@ -280,6 +281,7 @@ template <typename T>
 static int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hipStream_t s) {
     ZArgs z;
     z.Lx = p->Lx;
+    z.Lxs = p->Lxs;
     z.Ly = p->Ly;
     z.n3m = p->d.n3m;
     z.jlo = p->zjlo;
@@ -343,6 +345,9 @@ static int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hi
         case 3: part(integral_constant<int, 32>{}, integral_constant<int, 16>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
         case 4: part(integral_constant<int, 32>{}, integral_constant<int, 8>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
         case 5: part(integral_constant<int, 8>{}, integral_constant<int, 8>{}, integral_constant<int, 512>{}, integral_constant<int, 4>{}); break;
+        case 12: part(integral_constant<int, 24>{}, integral_constant<int, 16>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
+        case 13: part(integral_constant<int, 24>{}, integral_constant<int, 8>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
+        case 14: part(integral_constant<int, 48>{}, integral_constant<int, 8>{}, integral_constant<int, 512>{}, integral_constant<int, 1>{}); break;
         default: break;
     }
     if (!done) {
@@ -424,7 +429,8 @@ int ppf_plan_create(const ppf_desc *desc, ppf_plan **out) {
 
     int rc = PPF_OK;
     const bool fp32 = d.precision == PPF_PREC_FP32_IR;
-    const size_t wsz = (size_t)p->Lx * p->Ly * p->Lz;
+    p->Lxs = w_stride(p->Lx, fp32 ? (int)sizeof(float) : (int)sizeof(double));
+    const size_t wsz = (size_t)p->Lxs * p->Ly * p->Lz;
     // Thomas scratch only where the one-thread-per-line kernel can run: long
     // lines (n3m > 2048) or when it is selected explicitly
     const bool need_cs = d.n3m > 2048 || p->zvariant == 6;
@@ -525,7 +531,7 @@ static int run_passes_slabs(ppf_plan *p, T *W, const AxisTables<T> &tx, const Ax
                             double *P, bool acc, const double *dmx1, const double *dmx2, const double *dmx3,
                             const double *dx3, hipStream_t s) {
     const int ns = p->nslab, n3m = p->d.n3m;
-    const size_t wplane = (size_t)p->Lx * p->Ly, pplane = (size_t)(p->d.n1m + 2) * p->Ly;
+    const size_t wplane = (size_t)p->Lxs * p->Ly, pplane = (size_t)(p->d.n1m + 2) * p->Ly;
     const size_t fplane = (size_t)p->d.n1m * p->d.n2m;
     hipEvent_t *e = p->pev;
     int rc;
@@ -713,7 +719,7 @@ int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1
                   const double *dmx3, const double *dx3, void *stream, int *zvariant) {
     if (!p || !prhs || !P || !dmx1 || !dmx2 || !dmx3 || !dx3) return PPF_ERR_INVALID_ARG;
     // candidates: partitioned (K,S) shapes and the pipelined wavefront-PCR form
-    static const int cand[] = {3, 1, 2, 4, 5, 7, 8};
+    static const int cand[] = {3, 1, 2, 4, 5, 7, 8, 12, 13, 14};
     const int keep = p->zvariant;
     int best = 0;
     float best_ms = 0.f;
@@ -736,7 +742,7 @@ int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1
 }
 
 int ppf_plan_set_zvariant(ppf_plan *p, int zvariant) {
-    if (!p || zvariant < 0 || zvariant > 11) return PPF_ERR_INVALID_ARG;
+    if (!p || zvariant < 0 || zvariant > 14) return PPF_ERR_INVALID_ARG;
     p->zvariant = zvariant;
     return PPF_OK;
 }
