@@ -763,6 +763,8 @@ k_zthomas(T *__restrict__ W, T *__restrict__ cs, ZArgs z, const double *__restri
 #ifndef PPF_RCP_NEWTON
 #define PPF_RCP_NEWTON 2
 #endif
+// FP32: the hardware reciprocal is correct to about 1 ulp
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double frcp(double x) {
     // reciprocal: hardware estimate refined by Newton steps (each doubles the
     // correct bits; two give about 1 ulp)
@@ -799,68 +801,74 @@ k_zcoef(const double *__restrict__ dx3, const double *__restrict__ dmx3, int n3m
 
 // WPE: minimum waves per SIMD the register allocation must allow (occupancy
 // is what lets one workgroup's loads overlap another's arithmetic here).
-template <typename T, int K, int S, int NTMAX, int WPE>
+// CT: arithmetic type (FP64, or FP32 for the FP32 storage of refinement
+// plans); the workgroup of tile tile_base >= 0 alone, else blockIdx.x with
+// tile tile_skip left out (the FP32 launch leaves the (0,0) mode's tile to
+// an FP64 launch).
+template <typename T, int K, int S, int NTMAX, int WPE, typename CT = double>
 __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double *__restrict__ zap,
-        const double *__restrict__ dmx3, double *__restrict__ mean_out, int P) {
+        const double *__restrict__ dmx3, double *__restrict__ mean_out, int P, int tile_base = -1,
+        int tile_skip = 1 << 30) {
     extern __shared__ double zsh[];
     const int n3m = z.n3m;
     const int NP = P * K;           // padded rows
     double *am = zsh;               // [NP], 0 past n3m
     double *ap = am + NP;           // [NP], 0 past n3m
-    double *rl = ap + NP;           // reduced lower   [2P][S]
-    double *ru = rl + 2 * P * S;    // reduced upper
-    double *rd = ru + 2 * P * S;    // reduced rhs / solution
-    double *ps = rd + 2 * P * S;    // reduced diagonal [2P][S], later partial sums [P][S]
+    CT *rl = reinterpret_cast<CT *>(ap + NP);  // reduced lower   [2P][S]
+    CT *ru = rl + 2 * P * S;        // reduced upper
+    CT *rd = ru + 2 * P * S;        // reduced rhs / solution
+    CT *ps = rd + 2 * P * S;        // reduced diagonal [2P][S], later partial sums [P][S]
     const int tid = threadIdx.x, s = tid % S, p = tid / S;
     const int tiles = (z.Lx + S - 1) / S;
-    const int jp = z.jlo + blockIdx.x / tiles;
-    const int kx = (blockIdx.x % tiles) * S + s;
+    const int blk = tile_base >= 0 ? tile_base : (int)blockIdx.x + ((int)blockIdx.x >= tile_skip ? 1 : 0);
+    const int jp = z.jlo + blk / tiles;
+    const int kx = (blk % tiles) * S + s;
     const bool active = kx < z.Lx;
     const size_t pstride = (size_t)z.Ly * z.Lxs;
     T *w = W + (size_t)jp * z.Lxs + (active ? kx : 0);
-    const double lx = active ? z.lamx[kx] : 0.0;
-    const double ly = z.lamy[jp];
+    const CT lx = active ? (CT)z.lamx[kx] : CT(0);
+    const CT ly = (CT)z.lamy[jp];
     const bool pin = active && z.zx[kx] && z.zy[jp];
     const int g0 = p * K;
     // loads are clamped in bounds and selected, so the sweep code is branch-free
     const int glast = n3m - 1;
     // all loads first (clamped addresses, no consumer), masks afterwards
-    double d[K];
+    CT d[K];
 #pragma unroll
     for (int l = 0; l < K; ++l) {
         const int g = g0 + l;
-        d[l] = (double)ld_stream(&w[(size_t)((g < glast ? g : glast) + 1) * pstride]);
+        d[l] = (CT)ld_stream(&w[(size_t)((g < glast ? g : glast) + 1) * pstride]);
     }
 #pragma unroll
-    for (int l = 0; l < K; ++l) d[l] = (active && g0 + l < n3m) ? d[l] : 0.0;
-    if (pin && g0 == 0) d[0] = 0.0;
+    for (int l = 0; l < K; ++l) d[l] = (active && g0 + l < n3m) ? d[l] : CT(0);
+    if (pin && g0 == 0) d[0] = CT(0);
     // row coefficients (precomputed once per solve by k_zcoef) go to LDS while
     // the line loads above are still in flight
     for (int k = tid; k < NP; k += blockDim.x) {
         am[k] = k < n3m ? zam[k] : 0.0;
         ap[k] = k < n3m ? zap[k] : 0.0;
     }
-    double bc[4];  // z wall extrapolation, also precomputed by k_zcoef
+    CT bc[4];  // z wall extrapolation, also precomputed by k_zcoef
 #pragma unroll
-    for (int q = 0; q < 4; ++q) bc[q] = zap[n3m + q];
+    for (int q = 0; q < 4; ++q) bc[q] = (CT)zap[n3m + q];
     __syncthreads();
     // identity rows: padding past n3m, and the pinned first row of the (0,0) mode
-    auto coef = [&](int l, double &a, double &b, double &c) {
+    auto coef = [&](int l, CT &a, CT &b, CT &c) {
         const int g = g0 + l;
         const bool id = (g >= n3m) || (pin && g == 0);
-        const double am_ = am[g], ap_ = ap[g];
+        const CT am_ = (CT)am[g], ap_ = (CT)ap[g];
         a = am_;
-        c = id ? 0.0 : ap_;
-        b = id ? 1.0 : (-am_ - ap_ - lx) - ly;
+        c = id ? CT(0) : ap_;
+        b = id ? CT(1) : (-am_ - ap_ - lx) - ly;
     };
     // Reverse sweep (first-row equation) and forward sweep (last-row
     // equation) interleaved: two independent dependency chains per step.
     // The forward pivots r_l are kept for the final pass.
-    double rr[K];
-    double ah, ch, dh, Af, Df, Cf;
+    CT rr[K];
+    CT ah, ch, dh, Af, Df, Cf;
     {
-        double a, b, c;
+        CT a, b, c;
         coef(K - 2, a, b, c);
         double r = frcp(b);
         ah = r * a; ch = r * c; dh = r * d[K - 2];
@@ -872,7 +880,7 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
         for (int i = 1; i <= K - 2; ++i) {
             const int lb = K - 2 - i;  // reverse row
             const int lf = 1 + i;      // forward row
-            double a1, b1, c1, a2, b2, c2;
+            CT a1, b1, c1, a2, b2, c2;
             coef(lb, a1, b1, c1);
             coef(lf, a2, b2, c2);
             const double r1 = frcp(b1 - c1 * ah);
@@ -896,22 +904,22 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     rl[(2 * p + 1) * S + s] = Af;
     ru[(2 * p + 1) * S + s] = Cf;
     rd[(2 * p + 1) * S + s] = Df;
-    double *rb = ps;  // diagonal during the reduction
-    rb[(2 * p) * S + s] = 1.0;
-    rb[(2 * p + 1) * S + s] = 1.0;
+    CT *rb = ps;  // diagonal during the reduction
+    rb[(2 * p) * S + s] = CT(1);
+    rb[(2 * p + 1) * S + s] = CT(1);
     __syncthreads();
     // parallel cyclic reduction: every thread owns rows 2p and 2p+1 of its
     // column; ceil(log2 2P) steps, rows outside [0, 2P) are identity rows
     {
         const int R = 2 * P;
         for (int h = 1; h < R; h <<= 1) {
-            double nl[2], nb[2], nu[2], nd[2];
+            CT nl[2], nb[2], nu[2], nd[2];
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 const int i = 2 * p + q;
-                const double li = rl[i * S + s], bi = rb[i * S + s], ui = ru[i * S + s], di = rd[i * S + s];
-                double al = 0.0, ga = 0.0;
-                double lm = 0.0, um = 0.0, dm = 0.0, lp = 0.0, up = 0.0, dpp = 0.0;
+                const CT li = rl[i * S + s], bi = rb[i * S + s], ui = ru[i * S + s], di = rd[i * S + s];
+                CT al = 0, ga = 0;
+                CT lm = 0, um = 0, dm = 0, lp = 0, up = 0, dpp = 0;
                 if (i - h >= 0) {
                     const int j = i - h;
                     al = -li * frcp(rb[j * S + s]);
@@ -939,20 +947,20 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
             __syncthreads();
         }
     }
-    const double x0 = rd[(2 * p) * S + s] * frcp(rb[(2 * p) * S + s]);
-    const double xl = rd[(2 * p + 1) * S + s] * frcp(rb[(2 * p + 1) * S + s]);
+    const CT x0 = rd[(2 * p) * S + s] * frcp(rb[(2 * p) * S + s]);
+    const CT xl = rd[(2 * p + 1) * S + s] * frcp(rb[(2 * p + 1) * S + s]);
     __syncthreads();
     rd[(2 * p) * S + s] = x0;  // partition end values, read by the ghost code
     rd[(2 * p + 1) * S + s] = xl;
     __syncthreads();  // rb (= ps) is free from here on
     // interior rows 1..K-2 with both ends known: forward elimination with the
     // stored pivots, back substitution with C_l = r_l c_l (results in d)
-    double *x = d;
+    CT *x = d;
     {
-        double e = x0;
+        CT e = x0;
 #pragma unroll
         for (int l = 1; l <= K - 2; ++l) {
-            double a, b, c;
+            CT a, b, c;
             coef(l, a, b, c);
             e = rr[l] * (d[l] - a * e);
             d[l] = e;
@@ -962,13 +970,13 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
         x[K - 1] = xl;
 #pragma unroll
         for (int l = K - 2; l >= 1; --l) {
-            double a, b, c;
+            CT a, b, c;
             coef(l, a, b, c);
             x[l] = d[l] - rr[l] * c * x[l + 1];
             __builtin_amdgcn_sched_barrier(0);
         }
     }
-    double part = 0.0;
+    double part = 0.0;  // mean: FP64 (only the (0,0) tile's sum is used)
 #pragma unroll
     for (int l = 0; l < K; ++l) {
         const int g = g0 + l;
@@ -986,7 +994,7 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
             else w[0] = (T)(bc[0] * x[0] - bc[1] * x[1]);
         }
         if (p == pt) {
-            double xt = 0.0, xb = 0.0;
+            CT xt = 0, xb = 0;
 #pragma unroll
             for (int l = 0; l < K; ++l) {
                 if (l == lt) xt = x[l];
@@ -998,8 +1006,8 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
         }
     }
     // mean of the solution from the (0,0) real column (:628-664)
-    const bool mean_tile = (jp == z.mean_jp) && (blockIdx.x % tiles) * S <= z.mean_kx &&
-                           z.mean_kx < (blockIdx.x % tiles) * S + S;
+    const bool mean_tile = (jp == z.mean_jp) && (blk % tiles) * S <= z.mean_kx &&
+                           z.mean_kx < (blk % tiles) * S + S;
     if (mean_tile) {
         ps[p * S + s] = part;
         __syncthreads();
