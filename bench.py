@@ -13,7 +13,7 @@ residual against the FP64 solve plus its certification residual, and the FP64
 CPU oracle timed on a slab of the same workload on the host cores as a
 reported baseline.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--n 512] [--no-cpu] [--no-ir]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 10] [--n 512] [--no-cpu] [--no-ir]
 """
 from __future__ import annotations
 
@@ -275,7 +275,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10,
+                    help="untimed solves first: the first ~10 launches of each pass run slower while the GPU warms up")
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-ir", action="store_true", help="skip the FP32+refinement comparison")

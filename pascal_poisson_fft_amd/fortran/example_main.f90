@@ -14,7 +14,10 @@
 !> global_inputpara (global.f90:59-120).  Output: timing and the relative FP64
 !> residual on stdout; with ContinueFileout, dir_cont_fileout//'cont_P.bin'
 !> holds the interior P(1:n1m,1:n2m,1:n3m) as raw FP64 in Fortran order, the
-!> layout of cuda_Post_FileOut_Continue_Post_Reassembly_IO (cuda_post.f90:260-292).
+!> layout of cuda_Post_FileOut_Continue_Post_Reassembly_IO (cuda_post.f90:260-292);
+!> dir_instantfield//'P_000.PLT' is the Tecplot ASCII field of
+!> cuda_Post_Fileout_InstantField_3D (cuda_post.f90:294-331): P interpolated
+!> to the grid nodes, one "X Y Z P" line per node in E11.4.
 program ppf_example
     use iso_fortran_env, only : int64
     use fft_poisson
@@ -147,6 +150,8 @@ program ppf_example
         write(*, '(a)') 'Write continue file using post reassembly IO'
     end if
 
+    call write_instant_field(trim(dir_instantfield), P, x1, x2, x3, dx1, dx2, dx3)
+
     call cuda_Poisson_FFT_clean()
     call ppf_device_free_3d(PRHS_d)
     call ppf_device_free_3d(P_d)
@@ -157,6 +162,39 @@ program ppf_example
     write(*, '(a)') '[Main] Simulation complete. (GPU)'
 
 contains
+
+    !> Tecplot ASCII output of the reference example (cuda_post.f90:294-331,
+    !> calc_C2g :339-353): the value at node (x1(i), x2(j), x3(k)), i = 1..n1m
+    !> etc., is the cell-centred P interpolated linearly in z, then x, then y
+    !> over the 8 cells that share the node, each cell weighted by the width of
+    !> the cell on the other side of the node.
+    subroutine write_instant_field(dir, P, x1, x2, x3, dx1, dx2, dx3)
+        character(len=*), intent(in) :: dir
+        real(rp), intent(in) :: P(0:, 0:, 0:), x1(0:), x2(0:), x3(0:), dx1(0:), dx2(0:), dx3(0:)
+        integer :: u, i, j, k, nx, ny, nz
+        real(rp) :: wz0, wz1, wx0, wx1, wy0, wy1, lo, hi
+        nx = size(P, 1) - 2; ny = size(P, 2) - 2; nz = size(P, 3) - 2
+        call execute_command_line('mkdir -p '//dir)
+        open(newunit=u, file=dir//'P_000.PLT', status='replace', action='write')
+        write(u, '(a)') ' VARIABLES="X","Y","Z","P"'
+        write(u, '(a,i0,a,i0,a,i0,a,i0)') ' zone t="', 1, '" i=', nx, ' j=', ny, ' k=', nz
+        do k = 1, nz
+            ! weight of plane k is the width of plane k-1, and vice versa
+            wz1 = dx3(k - 1) / (dx3(k) + dx3(k - 1)); wz0 = dx3(k) / (dx3(k) + dx3(k - 1))
+            do j = 1, ny
+                wy1 = dx2(j - 1) / (dx2(j) + dx2(j - 1)); wy0 = dx2(j) / (dx2(j) + dx2(j - 1))
+                do i = 1, nx
+                    wx1 = dx1(i - 1) / (dx1(i) + dx1(i - 1)); wx0 = dx1(i) / (dx1(i) + dx1(i - 1))
+                    lo = wx1 * (wz1 * P(i, j - 1, k) + wz0 * P(i, j - 1, k - 1)) &
+                       + wx0 * (wz1 * P(i - 1, j - 1, k) + wz0 * P(i - 1, j - 1, k - 1))
+                    hi = wx1 * (wz1 * P(i, j, k) + wz0 * P(i, j, k - 1)) &
+                       + wx0 * (wz1 * P(i - 1, j, k) + wz0 * P(i - 1, j, k - 1))
+                    write(u, '(4(E11.4,1x))') x1(i), x2(j), x3(k), wy1 * hi + wy0 * lo
+                end do
+            end do
+        end do
+        close(u)
+    end subroutine write_instant_field
 
     !> Nodes, widths and centre spacings of one axis on one process:
     !> calculate_x / uniform_grid / non_uniform_channel_grid /

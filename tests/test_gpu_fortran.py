@@ -5,6 +5,11 @@ input in the reference's format, and its continue file is checked against the
 FP64 oracle on the same RHS and grid.
 
 Tolerance: relative L2 of the interior solution against the oracle <= 1e-12.
+The Tecplot file P_000.PLT (cuda_post.f90:294-331) holds node coordinates and
+P linearly interpolated to the nodes, printed in E11.4 (4 significant digits):
+each value must match the oracle's, interpolated the same way, to half a unit
+in the 4th digit (plus 1e-9 of the largest value, for nodes where P crosses
+zero).
 """
 import os
 import subprocess
@@ -71,6 +76,40 @@ dir_instantfield = './data/2_instantfield/'
 """
 
 
+def _node_weights(dx, n):
+    """Linear interpolation from cell centres i-1, i to node i (i = 1..n):
+    (weight of cell i, weight of cell i-1) = (dx_{i-1}, dx_i) / (dx_{i-1} + dx_i)."""
+    lo, hi = dx[0:n], dx[1:n + 1]
+    return lo / (lo + hi), hi / (lo + hi)
+
+
+def _check_instant_field(path, P, ax, ay, az):
+    n3, n2, n1 = (s - 2 for s in P.shape)
+    lines = path.read_text().splitlines()
+    assert lines[0].strip() == 'VARIABLES="X","Y","Z","P"'
+    assert lines[1].split()[-3:] == [f"i={n1}", f"j={n2}", f"k={n3}"]
+    got = np.array([[float(v) for v in ln.split()] for ln in lines[2:]])
+    assert got.shape == (n1 * n2 * n3, 4)
+    wx1, wx0 = _node_weights(ax.dx, n1)
+    wy1, wy0 = _node_weights(ay.dx, n2)
+    wz1, wz0 = _node_weights(az.dx, n3)
+    # P is [k][j][i] with ghosts; node (i,j,k) uses cells i-1..i, j-1..j, k-1..k
+    node = np.zeros((n3, n2, n1))
+    for cz, wz in ((1, wz1), (0, wz0)):
+        for cy, wy in ((1, wy1), (0, wy0)):
+            for cx, wx in ((1, wx1), (0, wx0)):
+                node += (wz[:, None, None] * wy[None, :, None] * wx[None, None, :]
+                         * P[cz:cz + n3, cy:cy + n2, cx:cx + n1])
+    k, j, i = np.meshgrid(np.arange(n3), np.arange(n2), np.arange(n1), indexing="ij")
+    coords = np.stack([ax.x[1:n1 + 1][i], ay.x[1:n2 + 1][j], az.x[1:n3 + 1][k]], axis=-1).reshape(-1, 3)
+    for col, exp in ((0, coords[:, 0]), (1, coords[:, 1]), (2, coords[:, 2]), (3, node.reshape(-1))):
+        # printing rounds to 4 significant digits; the floor covers values at
+        # rounding level (nodes where the solution crosses zero)
+        tol = 5.0e-4 * np.abs(exp) + 1e-9 * float(np.abs(exp).max())
+        bad = np.abs(got[:, col] - exp) > tol
+        assert not bad.any(), (col, int(bad.sum()), got[bad, col][:3], exp[bad][:3])
+
+
 def _flag(b):
     return ".true." if b else ".false."
 
@@ -109,9 +148,11 @@ def test_fortran_driver_matches_oracle(tmp_path, n1m, n2m, n3m, pbc1, pbc2, unif
     err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
     assert err <= 1e-12, err
 
+    P_full = O.solve(f, L, (pbc1, pbc2, False), ax.dmx, ay.dmx, az.dmx, az.dx)
+    _check_instant_field(tmp_path / "data" / "2_instantfield" / "P_000.PLT", P_full, ax, ay, az)
+
     res = [ln for ln in r.stdout.splitlines() if "Relative residual" in ln]
     assert res, r.stdout
     rel = float(res[0].split(":")[1])
-    P_ref = O.solve(f, L, (pbc1, pbc2, False), ax.dmx, ay.dmx, az.dmx, az.dx)
-    rel_ref = O.rel_residual(f, P_ref, L, (pbc1, pbc2, False), az.dmx, az.dx)
+    rel_ref = O.rel_residual(f, P_full, L, (pbc1, pbc2, False), az.dmx, az.dx)
     assert rel <= max(1e-11, 10.0 * rel_ref), (rel, rel_ref)
