@@ -753,8 +753,10 @@ k_zthomas(T *__restrict__ W, T *__restrict__ cs, ZArgs z, const double *__restri
 //       A_l x_0 + x_l + C_l x_{l+1} = D_l,  row K-1 couples to x_K
 // These two rows of every partition form a 2P-row tridiagonal system in the
 // partition end values (the reduced system of PaScaL_TDMA's modified Thomas,
-// pascal_tdma_cuda.f90:332-473 and :230-269 for many ranks), solved per column
-// in LDS.  With x_0 and x_{K-1} known, a Thomas sweep over rows 1..K-2 that
+// pascal_tdma_cuda.f90:332-473 and :230-269 for many ranks).  Substituting the
+// first-row equations leaves P rows in the last values x_{K-1}, solved per
+// column by cyclic reduction in LDS; x_0 follows from its first-row
+// equation.  With x_0 and x_{K-1} known, a Thomas sweep over rows 1..K-2 that
 // reuses the forward multipliers C_l recovers the interior.  Both reduction
 // sweeps carry O(1) state, so a thread holds 2K values (its right-hand sides
 // and C).  Rows past n3m are identity rows.  Coefficients, pin, ghost planes
@@ -815,10 +817,11 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     const int NP = P * K;           // padded rows
     double *am = zsh;               // [NP], 0 past n3m
     double *ap = am + NP;           // [NP], 0 past n3m
-    CT *rl = reinterpret_cast<CT *>(ap + NP);  // reduced lower   [2P][S]
-    CT *ru = rl + 2 * P * S;        // reduced upper
-    CT *rd = ru + 2 * P * S;        // reduced rhs / solution
-    CT *ps = rd + 2 * P * S;        // reduced diagonal [2P][S], later partial sums [P][S]
+    const int PS = P * S;
+    CT *ra = reinterpret_cast<CT *>(ap + NP);  // [3][P][S]: first-row equations, then reduction buffer
+    CT *rc = ra + 3 * PS;           // [3][P][S]: reduction buffer
+    CT *xe = rc + 3 * PS;           // [P][S]: partition end values x_{K-1}
+    CT *ps = ra;                    // [P][S]: partial sums, once the reduction is done
     const int tid = threadIdx.x, s = tid % S, p = tid / S;
     const int tiles = (z.Lx + S - 1) / S;
     const int blk = tile_base >= 0 ? tile_base : (int)blockIdx.x + ((int)blockIdx.x >= tile_skip ? 1 : 0);
@@ -896,63 +899,62 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
             __builtin_amdgcn_sched_barrier(0);
         }
     }
-    // reduced system: rows 2p (first-row equation) and 2p+1 (last-row
-    // equation) of column s; unit diagonal
-    rl[(2 * p) * S + s] = ah;
-    ru[(2 * p) * S + s] = ch;
-    rd[(2 * p) * S + s] = dh;
-    rl[(2 * p + 1) * S + s] = Af;
-    ru[(2 * p + 1) * S + s] = Cf;
-    rd[(2 * p + 1) * S + s] = Df;
-    CT *rb = ps;  // diagonal during the reduction
-    rb[(2 * p) * S + s] = CT(1);
-    rb[(2 * p + 1) * S + s] = CT(1);
+    // Reduced system in the partition end values e_p = x_{K-1} of partition p.
+    // The first-row equation gives x_0 = dh - ah e_{p-1} - ch e_p; put into the
+    // last-row equation Af x_0 + e_p + Cf x_0(p+1) = Df it leaves one
+    // tridiagonal row per partition,
+    //   (-Af ah_p) e_{p-1} + (1 - Af ch_p - Cf ah_{p+1}) e_p + (-Cf ch_{p+1}) e_{p+1}
+    //     = Df - Af dh_p - Cf dh_{p+1},
+    // kept with a unit diagonal as (el, eu, er).
+    const int o = p * S + s;
+    ra[o] = ah;
+    ra[PS + o] = ch;
+    ra[2 * PS + o] = dh;
     __syncthreads();
-    // parallel cyclic reduction: every thread owns rows 2p and 2p+1 of its
-    // column; ceil(log2 2P) steps, rows outside [0, 2P) are identity rows
+    CT el, eu, er;
     {
-        const int R = 2 * P;
-        for (int h = 1; h < R; h <<= 1) {
-            CT nl[2], nb[2], nu[2], nd[2];
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const int i = 2 * p + q;
-                const CT li = rl[i * S + s], bi = rb[i * S + s], ui = ru[i * S + s], di = rd[i * S + s];
-                CT al = 0, ga = 0;
-                CT lm = 0, um = 0, dm = 0, lp = 0, up = 0, dpp = 0;
-                if (i - h >= 0) {
-                    const int j = i - h;
-                    al = -li * frcp(rb[j * S + s]);
-                    lm = rl[j * S + s]; um = ru[j * S + s]; dm = rd[j * S + s];
-                }
-                if (i + h < R) {
-                    const int j = i + h;
-                    ga = -ui * frcp(rb[j * S + s]);
-                    lp = rl[j * S + s]; up = ru[j * S + s]; dpp = rd[j * S + s];
-                }
-                nb[q] = bi + al * um + ga * lp;
-                nl[q] = al * lm;
-                nu[q] = ga * up;
-                nd[q] = di + al * dm + ga * dpp;
-            }
+        CT ah1 = 0, ch1 = 0, dh1 = 0;
+        if (p + 1 < P) {
+            ah1 = ra[o + S];
+            ch1 = ra[PS + o + S];
+            dh1 = ra[2 * PS + o + S];
+        }
+        const CT inv = frcp(CT(1) - Af * ch - Cf * ah1);
+        el = -Af * ah * inv;
+        eu = -Cf * ch1 * inv;
+        er = (Df - Af * dh - Cf * dh1) * inv;
+    }
+    // parallel cyclic reduction over the P rows of each column: step h couples
+    // row p to p +- 2h; rows outside [0, P) are zero.  The rows alternate
+    // between two LDS buffers, so one barrier per step separates every write
+    // from the reads of the same buffer.
+    {
+        CT *buf = rc;
+        for (int h = 1; h < P; h <<= 1) {
+            buf[o] = el;
+            buf[PS + o] = eu;
+            buf[2 * PS + o] = er;
             __syncthreads();
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const int i = 2 * p + q;
-                rl[i * S + s] = nl[q];
-                rb[i * S + s] = nb[q];
-                ru[i * S + s] = nu[q];
-                rd[i * S + s] = nd[q];
+            CT lm = 0, um = 0, rm = 0, lp = 0, up = 0, rp = 0;
+            if (p >= h) {
+                const int q = o - h * S;
+                lm = buf[q]; um = buf[PS + q]; rm = buf[2 * PS + q];
             }
-            __syncthreads();
+            if (p + h < P) {
+                const int q = o + h * S;
+                lp = buf[q]; up = buf[PS + q]; rp = buf[2 * PS + q];
+            }
+            const CT inv = frcp(CT(1) - el * um - eu * lp);
+            er = (er - el * rm - eu * rp) * inv;
+            el = -el * lm * inv;
+            eu = -eu * up * inv;
+            buf = (buf == rc) ? ra : rc;
         }
     }
-    const CT x0 = rd[(2 * p) * S + s] * frcp(rb[(2 * p) * S + s]);
-    const CT xl = rd[(2 * p + 1) * S + s] * frcp(rb[(2 * p + 1) * S + s]);
-    __syncthreads();
-    rd[(2 * p) * S + s] = x0;  // partition end values, read by the ghost code
-    rd[(2 * p + 1) * S + s] = xl;
-    __syncthreads();  // rb (= ps) is free from here on
+    const CT xl = er;
+    xe[o] = xl;
+    __syncthreads();  // xe complete; the reduction buffers (and ps) are free
+    const CT x0 = dh - ah * (p > 0 ? xe[o - S] : CT(0)) - ch * xl;
     // interior rows 1..K-2 with both ends known: forward elimination with the
     // stored pivots, back substitution with C_l = r_l c_l (results in d)
     CT *x = d;
@@ -1000,7 +1002,7 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
                 if (l == lt) xt = x[l];
                 if (l == lt - 1) xb = x[l];
             }
-            if (lt == 0) xb = rd[(2 * (pt - 1) + 1) * S + s];
+            if (lt == 0) xb = xe[(pt - 1) * S + s];
             if (z.zper) w[0] = (T)xt;
             else w[(size_t)(n3m + 1) * pstride] = (T)(bc[2] * xt - bc[3] * xb);
         }
