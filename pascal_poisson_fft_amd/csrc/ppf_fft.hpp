@@ -178,8 +178,26 @@ __device__ __forceinline__ void fft_stage(LLine<T> line, int t, const cx<T> *__r
             const int j = b % LS;
             const int kk = b / LS;
             if (LS > 1) {
+                const cx<T> *w = tw + (LS - 1) + j;  // w[(r - 1) * LS] = exp(-2 pi i j r / (LS R))
+                if constexpr (sizeof(T) == 8 && (R == 8 || R == 4)) {
+                    // FP64: powers 3, 5, 6, 7 as products of loaded powers 1, 2, 4
+                    // (a few ulp), trading LDS reads shared by the CU for VALU work
+                    const cx<T> w1 = w[0], w2 = w[LS];
+                    const cx<T> w3 = cmul(w1, w2);
+                    v[q][1] = cmul(v[q][1], w1);
+                    v[q][2] = cmul(v[q][2], w2);
+                    v[q][3] = cmul(v[q][3], w3);
+                    if constexpr (R == 8) {
+                        const cx<T> w4 = w[3 * LS];
+                        v[q][4] = cmul(v[q][4], w4);
+                        v[q][5] = cmul(v[q][5], cmul(w1, w4));
+                        v[q][6] = cmul(v[q][6], cmul(w2, w4));
+                        v[q][7] = cmul(v[q][7], cmul(w3, w4));
+                    }
+                } else {
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], tw[(LS - 1) + (r - 1) * LS + j]);
+                    for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], w[(r - 1) * LS]);
+                }
             }
             Dft<T, R>::run(v[q]);
 #pragma unroll

@@ -125,6 +125,37 @@ __device__ __forceinline__ int fdiv_small(int a, int b, float rb) {
 // cells, x the ghosted FP64 iterate.  L is the operator the solve inverts:
 // uniform x/y spacing with zero-flux walls (the DCT eigenvalues) or wrap, and
 // the stretched z rows zam/zap of fft_3d_poisson.f90:1143-1146.
+// z row k (0-based) of the stretched operator, fft_3d_poisson.f90:1143-1146:
+// am couples to row k-1 (0 on the bottom row), ap to row k+1 (0 on the top).
+__device__ __forceinline__ void z_row(const double *__restrict__ dx3, const double *__restrict__ dmx3, int n3m, int k,
+                                      double &am, double &ap) {
+    const int kk = k + 1;
+    am = kk == 1 ? 0.0 : 1.0 / dx3[kk] / dmx3[kk];
+    ap = kk == n3m ? 0.0 : 1.0 / dx3[kk] / dmx3[kk + 1];
+}
+// z wall extrapolation p0 = BC_a p1 - BC_b p2 (:829-837): bc[0..1] at the
+// bottom wall, bc[2..3] at the top wall.
+__device__ __forceinline__ void z_wall_bc(const double *__restrict__ dmx3, int n3m, double *bc) {
+    const double d1 = dmx3[1], d2 = dmx3[2], s0 = d1 + d2;
+    bc[0] = s0 * s0 / (s0 * s0 - d1 * d1);
+    bc[1] = d1 * d1 / (s0 * s0 - d1 * d1);
+    const double e1 = dmx3[n3m + 1], e2 = dmx3[n3m], s1 = e1 + e2;
+    bc[2] = s1 * s1 / (s1 * s1 - e1 * e1);
+    bc[3] = e1 * e1 / (s1 * s1 - e1 * e1);
+}
+
+// The per-solve z row table (zam[k], zap[k], then the 4 wall coefficients at
+// zap[n3m..]); pass 1 fills it (workgroup 0) when zam is set.
+struct ZRows {
+    const double *dx3, *dmx3;
+    double *zam, *zap;
+    int n3m;
+};
+__device__ __forceinline__ void fill_z_rows(const ZRows &zr, int tid, int nt) {
+    for (int k = tid; k < zr.n3m; k += nt) z_row(zr.dx3, zr.dmx3, zr.n3m, k, zr.zam[k], zr.zap[k]);
+    if (tid == 0) z_wall_bc(zr.dmx3, zr.n3m, zr.zap + zr.n3m);
+}
+
 struct XRes {
     const double *P;          // (n1m+2)(n2m+2)(n3m+2) iterate
     const double *zam, *zap;  // z rows (zero at the walls)
@@ -195,7 +226,8 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 template <typename T, int N, bool PER, bool RES = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RES ? 1 : PPF_XWPE)))
 k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, int Ly,
-       const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk, XRes xr) {
+       const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk, XRes xr,
+       ZRows zr) {
     using X = XGeom<T, N, PER>;
     using G = typename X::G;
     using L = typename X::L;
@@ -241,6 +273,8 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
     };
     int tile = blockIdx.x;
     if (tile < ntiles) fetch(tile);
+    // the z row table of this solve, for pass 3 (saves a launch)
+    if (zr.zam && blockIdx.x == 0) fill_z_rows(zr, tid, NT);
     L::load_tables(lds, tw, wk, qk, tid, NT);
     const cx<T> *tab = L::tables(lds);
     const LLine<T> my = L::line(lds, l);
@@ -779,26 +813,15 @@ __device__ __forceinline__ double frcp(double x) {
     return r;
 }
 
-// z row coefficients for one solve: zam[k], zap[k] (k < n3m) as in
-// fft_3d_poisson.f90:1143-1146, then zap[n3m..n3m+3] = BC_a, BC_b at the
-// bottom wall and at the top wall (:829-837).
+// z row coefficients for one solve as a table (read by the residual forms and
+// the pipelined z kernel): zam[k], zap[k] (k < n3m), then zap[n3m..n3m+3] =
+// the wall extrapolation coefficients.
 __global__ void __launch_bounds__(256)
 k_zcoef(const double *__restrict__ dx3, const double *__restrict__ dmx3, int n3m, double *__restrict__ zam,
         double *__restrict__ zap) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n3m) {
-        const int kk = k + 1;
-        zam[k] = kk == 1 ? 0.0 : 1.0 / dx3[kk] / dmx3[kk];
-        zap[k] = kk == n3m ? 0.0 : 1.0 / dx3[kk] / dmx3[kk + 1];
-    }
-    if (k == 0) {
-        const double d1 = dmx3[1], d2 = dmx3[2], s0 = d1 + d2;
-        zap[n3m] = s0 * s0 / (s0 * s0 - d1 * d1);
-        zap[n3m + 1] = d1 * d1 / (s0 * s0 - d1 * d1);
-        const double e1 = dmx3[n3m + 1], e2 = dmx3[n3m], s1 = e1 + e2;
-        zap[n3m + 2] = s1 * s1 / (s1 * s1 - e1 * e1);
-        zap[n3m + 3] = e1 * e1 / (s1 * s1 - e1 * e1);
-    }
+    if (k < n3m) z_row(dx3, dmx3, n3m, k, zam[k], zap[k]);
+    if (k == 0) z_wall_bc(dmx3, n3m, zap + n3m);
 }
 
 // WPE: minimum waves per SIMD the register allocation must allow (occupancy
@@ -810,8 +833,7 @@ k_zcoef(const double *__restrict__ dx3, const double *__restrict__ dmx3, int n3m
 template <typename T, int K, int S, int NTMAX, int WPE, typename CT = double>
 __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double *__restrict__ zap,
-        const double *__restrict__ dmx3, double *__restrict__ mean_out, int P, int tile_base = -1,
-        int tile_skip = 1 << 30) {
+        double *__restrict__ mean_out, int P, int tile_base = -1, int tile_skip = 1 << 30) {
     extern __shared__ double zsh[];
     const int n3m = z.n3m;
     const int NP = P * K;           // padded rows
@@ -846,13 +868,13 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
 #pragma unroll
     for (int l = 0; l < K; ++l) d[l] = (active && g0 + l < n3m) ? d[l] : CT(0);
     if (pin && g0 == 0) d[0] = CT(0);
-    // row coefficients (precomputed once per solve by k_zcoef) go to LDS while
+    // row coefficients (the per-solve table filled by pass 1) go to LDS while
     // the line loads above are still in flight
     for (int k = tid; k < NP; k += blockDim.x) {
         am[k] = k < n3m ? zam[k] : 0.0;
         ap[k] = k < n3m ? zap[k] : 0.0;
     }
-    CT bc[4];  // z wall extrapolation, also precomputed by k_zcoef
+    CT bc[4];  // z wall extrapolation, from the same table
 #pragma unroll
     for (int q = 0; q < 4; ++q) bc[q] = (CT)zap[n3m + q];
     __syncthreads();

@@ -166,25 +166,25 @@ static unsigned persistent_grid(K kernel, int nthreads, int ntiles, size_t dyn_l
 // sums is capped by the partial buffer
 template <typename T, int N, bool PER, bool RES>
 static unsigned launch_xfwd_k(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, const XRes &xr,
-                              hipStream_t s, int nplanes) {
+                              const ZRows &zr, hipStream_t s, int nplanes) {
     using X = XGeom<T, N, PER>;
     const int nlines = p->d.n2m * nplanes;
     unsigned g = persistent_grid(k_xfwd<T, N, PER, RES>, X::NT, (nlines + X::LPB - 1) / X::LPB);
     if (RES) g = std::min<unsigned>(g, (unsigned)p->npartial);
     hipLaunchKernelGGL((k_xfwd<T, N, PER, RES>), dim3(g), dim3(X::NT), 0, s, in, W, nlines, p->d.n2m, p->Ly, tb.tw,
-                       tb.wk, tb.qk, xr);
+                       tb.wk, tb.qk, xr, zr);
     return g;
 }
 
 template <typename T, int N>
 static unsigned launch_xfwd(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, const XRes *xr,
-                            hipStream_t s, int nplanes) {
+                            const ZRows &zr, hipStream_t s, int nplanes) {
     XRes none{};
     if (p->xkind == AX_R2C)
-        return xr ? launch_xfwd_k<T, N, true, true>(p, in, W, tb, *xr, s, nplanes)
-                  : launch_xfwd_k<T, N, true, false>(p, in, W, tb, none, s, nplanes);
-    return xr ? launch_xfwd_k<T, N, false, true>(p, in, W, tb, *xr, s, nplanes)
-              : launch_xfwd_k<T, N, false, false>(p, in, W, tb, none, s, nplanes);
+        return xr ? launch_xfwd_k<T, N, true, true>(p, in, W, tb, *xr, zr, s, nplanes)
+                  : launch_xfwd_k<T, N, true, false>(p, in, W, tb, none, zr, s, nplanes);
+    return xr ? launch_xfwd_k<T, N, false, true>(p, in, W, tb, *xr, zr, s, nplanes)
+              : launch_xfwd_k<T, N, false, false>(p, in, W, tb, none, zr, s, nplanes);
 }
 
 template <typename T, int N, bool PER, bool ACC>
@@ -236,16 +236,21 @@ static void launch_y(const ppf_plan *p, T *W, const AxisTables<T> &tb, bool inve
     else launch_y_mode<T, N, 2>(p, W, tb, inverse, dmx2, s, nplanes);
 }
 
+static ZRows z_rows(const ppf_plan *p, const double *dx3, const double *dmx3) {
+    return ZRows{dx3, dmx3, p->zcoef, p->zcoef + p->d.n3m, p->d.n3m};
+}
+
+// zr: when zr.zam is set, workgroup 0 also fills the plan's z row table
 template <typename T>
 static int dispatch_xfwd(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, const XRes *xr,
-                         hipStream_t s, unsigned *grid = nullptr, int nplanes = -1) {
+                         hipStream_t s, unsigned *grid = nullptr, int nplanes = -1, ZRows zr = ZRows{}) {
     unsigned g = 0;
     if (nplanes < 0) nplanes = p->d.n3m;
     switch (p->d.n1m) {
 #define X_CASE(n) \
-    case n:                                                     \
-        g = launch_xfwd<T, n>(p, in, W, tb, xr, s, nplanes);   \
-        if (grid) *grid = g;                                    \
+    case n:                                                         \
+        g = launch_xfwd<T, n>(p, in, W, tb, xr, zr, s, nplanes);   \
+        if (grid) *grid = g;                                        \
         return PPF_OK;
         PPF_FOR_SIZES(X_CASE)
 #undef X_CASE
@@ -279,6 +284,7 @@ static int dispatch_y(const ppf_plan *p, T *W, const AxisTables<T> &tb, bool inv
 
 template <typename T>
 static int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hipStream_t s) {
+    // k_zpart and k_zpipe read the z row table that pass 1 of this solve filled
     ZArgs z;
     z.Lx = p->Lx;
     z.Lxs = p->Lxs;
@@ -295,8 +301,6 @@ static int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hi
     z.mean_scale = p->mean_scale;
     z.zper = p->d.pbc3 != 0;
     const int n3m = p->d.n3m;
-    // z row coefficients of this solve (also read by the refinement residual)
-    hipLaunchKernelGGL(k_zcoef, dim3((n3m + 255) / 256), dim3(256), 0, s, dx3, dmx3, n3m, p->zcoef, p->zcoef + n3m);
     // partitioned register-resident Thomas, shape (K rows per thread, S
     // columns per workgroup); launched only when S*P <= NTMAX and the LDS
     // footprint stays within 64 KiB, else the one-thread-per-line kernel
@@ -308,12 +312,11 @@ static int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hi
         const size_t shm = sizeof(double) * (2 * (size_t)P * K + 7 * (size_t)P * S);
         if (P < 1 || S * P > NTMAX || shm > 65536) return;
         const int tiles = (p->Lx + S - 1) / S;
-        const double *zam = p->zcoef, *zap = p->zcoef + n3m;
         const int ntiles = tiles * p->znjr;
         // FP64 arithmetic for FP32 storage too: FP32 arithmetic measured no
         // faster at 512^3 (0.509 vs 0.515 ms) and costs the refinement a sweep
-        hipLaunchKernelGGL((k_zpart<T, K, S, NTMAX, WPE>), dim3((unsigned)ntiles), dim3(S * P), shm, s, W, z, zam, zap,
-                           dmx3, p->mean, P, -1, 1 << 30);
+        hipLaunchKernelGGL((k_zpart<T, K, S, NTMAX, WPE>), dim3((unsigned)ntiles), dim3(S * P), shm, s, W, z,
+                           p->zcoef, p->zcoef + n3m, p->mean, P, -1, 1 << 30);
         done = true;
     };
     // pipelined variant: persistent, prefetching, wavefront cyclic reduction
@@ -351,6 +354,7 @@ static int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hi
         case 12: part(integral_constant<int, 24>{}, integral_constant<int, 16>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
         case 13: part(integral_constant<int, 24>{}, integral_constant<int, 8>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
         case 14: part(integral_constant<int, 48>{}, integral_constant<int, 8>{}, integral_constant<int, 512>{}, integral_constant<int, 1>{}); break;
+        case 15: part(integral_constant<int, 32>{}, integral_constant<int, 32>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
         default: break;
     }
     if (!done) {
@@ -544,7 +548,7 @@ static int run_passes_slabs(ppf_plan *p, T *W, const AxisTables<T> &tx, const Ax
     for (int g = 0; g < ns; ++g) {
         const int k0 = (int)((long)g * n3m / ns), k1 = (int)((long)(g + 1) * n3m / ns);
         if ((rc = dispatch_xfwd<T>(p, in + (size_t)k0 * fplane, W + (size_t)k0 * wplane, tx, nullptr, s, nullptr,
-                                   k1 - k0)))
+                                   k1 - k0, g == 0 ? z_rows(p, dx3, dmx3) : ZRows{})))
             return rc;
         PPF_HIP(hipEventRecord(e[1 + g], s));
         PPF_HIP(hipStreamWaitEvent(p->s2, e[1 + g], 0));
@@ -581,7 +585,9 @@ static int run_passes(ppf_plan *p, T *W, const AxisTables<T> &tx, const AxisTabl
     int rc;
     if (p->nslab > 1 && !ev && !xr) return run_passes_slabs<T>(p, W, tx, ty, in, P, acc, dmx1, dmx2, dmx3, dx3, s);
     if (ev) PPF_HIP(hipEventRecord(ev[0], s));
-    if ((rc = dispatch_xfwd<T>(p, in, W, tx, xr, s, res_grid))) return rc;
+    // pass 1 also fills the z row table of this solve (not in the residual
+    // form, which reads it: the table of the solve that made the iterate)
+    if ((rc = dispatch_xfwd<T>(p, in, W, tx, xr, s, res_grid, -1, xr ? ZRows{} : z_rows(p, dx3, dmx3)))) return rc;
     if (ev) PPF_HIP(hipEventRecord(ev[1], s));
     if ((rc = dispatch_y<T>(p, W, ty, false, dmx2, s))) return rc;
     if (ev) PPF_HIP(hipEventRecord(ev[2], s));
@@ -745,7 +751,7 @@ int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1
 }
 
 int ppf_plan_set_zvariant(ppf_plan *p, int zvariant) {
-    if (!p || zvariant < 0 || zvariant > 14) return PPF_ERR_INVALID_ARG;
+    if (!p || zvariant < 0 || zvariant > 15) return PPF_ERR_INVALID_ARG;
     p->zvariant = zvariant;
     return PPF_OK;
 }
