@@ -488,10 +488,23 @@ struct YGeom {
     static_assert(NT <= kMaxNT, "y-pass workgroup exceeds kMaxNT threads");
     static_assert(lds_bytes(TX) <= 160 * 1024, "y-pass tile exceeds the CU's LDS");
     using L = LdsLines<T, M, S, NL>;
+    // Row-blocked mapping (DCT columns): thread (g, c2) owns the RPT
+    // consecutive rows RPT*g .. RPT*g+RPT-1 of column pair c2.  The Makhoul
+    // order then puts a thread's rows into 2*(RPT/4) complex LDS slots per
+    // column, known once per thread, and LDS is read and written as whole
+    // complex values (16 bytes in FP64).
+    static constexpr int TP = TX / 2;
+    static constexpr int G = NT / TP;
+    static constexpr int RPT = G > 0 ? N / G : 0;
+    static constexpr bool RB = MODE == 0 && NT % TP == 0 && G * RPT == N && RPT % 4 == 0;
+    // registers: uncapped (PPF_YWPE = 1).  The row-blocked 512-point FP64
+    // kernels hold 178-186 VGPRs; a cap of 128 (two 512-thread workgroups per
+    // CU) spills about 50 of them and measured 1.7x slower.
+    static constexpr int WPE = PPF_YWPE;
 };
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(PPF_YWPE)))
+__global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(YGeom<T, N, MODE>::WPE)))
 k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
        const cx<T> *__restrict__ qk) {
     using Y = YGeom<T, N, MODE>;
@@ -504,18 +517,28 @@ k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int nct = (Lx + TX - 1) / TX;
     const int ntiles = nct * nplanes;
-    V2 pre[EPT];
+    constexpr int RPT = Y::RPT, NPRE = Y::RB ? RPT : EPT;
+    const int bg = tid / TP, bc2 = tid - bg * TP;  // row-blocked: group, column pair
+    V2 pre[NPRE];
     auto fetch = [&](int tile) {  // unconditional, clamped: see k_xfwd
         const int ct = tile % nct, pl = tile / nct;
         const size_t plane = (size_t)(pl + 1) * Ly;
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            int e = tid + q * NT;
-            e = e < TP * N ? e : TP * N - 1;
-            const int r = e / TP, c2 = e - r * TP;
-            int col = ct * TX + 2 * c2;
+        if constexpr (Y::RB) {
+            int col = ct * TX + 2 * bc2;
             col = col < Lx ? col : Lx - 2;  // Lx is even
-            pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[(plane + 1 + r) * Lxs + col]));
+            const T *src = &W[(plane + 1 + (size_t)RPT * bg) * Lxs + col];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) pre[q] = ld_stream(reinterpret_cast<const V2 *>(src + (size_t)q * Lxs));
+        } else {
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                int e = tid + q * NT;
+                e = e < TP * N ? e : TP * N - 1;
+                const int r = e / TP, c2 = e - r * TP;
+                int col = ct * TX + 2 * c2;
+                col = col < Lx ? col : Lx - 2;  // Lx is even
+                pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[(plane + 1 + r) * Lxs + col]));
+            }
         }
     };
     int tile = blockIdx.x;
@@ -526,20 +549,35 @@ k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
     const int c0 = (tile % nct) * TX;
     const size_t plane = (size_t)(tile / nct + 1) * Ly;
     __syncthreads();  // the previous tile's LDS lines have been stored
+    if constexpr (Y::RB) {
+        // Makhoul order: row RPT g + 4j (+2) is component 0 (1) of complex slot
+        // (RPT/4) g + j; row RPT g + 4j + 3 (+1) is component 0 (1) of slot
+        // N/2 - 1 - (RPT/4) g - j
+        const LLine<T> c0l = L::line(lds, 2 * bc2), c1l = L::line(lds, 2 * bc2 + 1);
 #pragma unroll
-    for (int q = 0; q < EPT; ++q) {
-        const int e = tid + q * NT;
-        if (e < TP * N) {
-            const int r = e / TP, c2 = e - r * TP;
-            const V2 v = pre[q];
-            if (MODE == 0) {
-                L::real(lds, 2 * c2)[makhoul_pos<N>(r)] = v.x;
-                L::real(lds, 2 * c2 + 1)[makhoul_pos<N>(r)] = v.y;
-            } else if (MODE == 1) {  // a pair is one complex column
-                L::line(lds, c2)[r] = cx<T>{v.x, v.y};
-            } else {
-                L::real(lds, 2 * c2)[r] = v.x;
-                L::real(lds, 2 * c2 + 1)[r] = v.y;
+        for (int j = 0; j < RPT / 4; ++j) {
+            const int ce = (RPT / 4) * bg + j, co = N / 2 - 1 - (RPT / 4) * bg - j;
+            c0l[ce] = cx<T>{pre[4 * j].x, pre[4 * j + 2].x};
+            c1l[ce] = cx<T>{pre[4 * j].y, pre[4 * j + 2].y};
+            c0l[co] = cx<T>{pre[4 * j + 3].x, pre[4 * j + 1].x};
+            c1l[co] = cx<T>{pre[4 * j + 3].y, pre[4 * j + 1].y};
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = tid + q * NT;
+            if (e < TP * N) {
+                const int r = e / TP, c2 = e - r * TP;
+                const V2 v = pre[q];
+                if (MODE == 0) {
+                    L::real(lds, 2 * c2)[makhoul_pos<N>(r)] = v.x;
+                    L::real(lds, 2 * c2 + 1)[makhoul_pos<N>(r)] = v.y;
+                } else if (MODE == 1) {  // a pair is one complex column
+                    L::line(lds, c2)[r] = cx<T>{v.x, v.y};
+                } else {
+                    L::real(lds, 2 * c2)[r] = v.x;
+                    L::real(lds, 2 * c2 + 1)[r] = v.y;
+                }
             }
         }
     }
@@ -554,26 +592,41 @@ k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
     }
     constexpr int NR = MODE == 2 ? N + 2 : N;  // spectral rows written
     constexpr int R0 = MODE == 2 ? 0 : 1;
-    for (int e = tid; e < TP * NR; e += NT) {
-        const int r = e / TP, c2 = e - r * TP;
-        const int col = c0 + 2 * c2;
-        if (col >= Lx) continue;
-        V2 v;
-        if (MODE == 1) {
-            const cx<T> z = L::line(lds, c2)[r];
-            v.x = z.x;
-            v.y = z.y;
-        } else {
-            v.x = L::real(lds, 2 * c2)[r];
-            v.y = L::real(lds, 2 * c2 + 1)[r];
+    if constexpr (Y::RB) {
+        // natural order: rows RPT g + 2k, +1 are the components of slot (RPT/2) g + k
+        const int col = c0 + 2 * bc2;
+        if (col < Lx) {
+            const LLine<T> c0l = L::line(lds, 2 * bc2), c1l = L::line(lds, 2 * bc2 + 1);
+            T *dst = &W[(plane + R0 + (size_t)RPT * bg) * Lxs + col];
+#pragma unroll
+            for (int k = 0; k < RPT / 2; ++k) {
+                const cx<T> z0 = c0l[(RPT / 2) * bg + k], z1 = c1l[(RPT / 2) * bg + k];
+                st_stream(reinterpret_cast<V2 *>(dst + (size_t)(2 * k) * Lxs), V2{z0.x, z1.x});
+                st_stream(reinterpret_cast<V2 *>(dst + (size_t)(2 * k + 1) * Lxs), V2{z0.y, z1.y});
+            }
         }
-        st_stream(reinterpret_cast<V2 *>(&W[(plane + R0 + r) * Lxs + col]), v);
+    } else {
+        for (int e = tid; e < TP * NR; e += NT) {
+            const int r = e / TP, c2 = e - r * TP;
+            const int col = c0 + 2 * c2;
+            if (col >= Lx) continue;
+            V2 v;
+            if (MODE == 1) {
+                const cx<T> z = L::line(lds, c2)[r];
+                v.x = z.x;
+                v.y = z.y;
+            } else {
+                v.x = L::real(lds, 2 * c2)[r];
+                v.y = L::real(lds, 2 * c2 + 1)[r];
+            }
+            st_stream(reinterpret_cast<V2 *>(&W[(plane + R0 + r) * Lxs + col]), v);
+        }
     }
     }
 }
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(PPF_YWPE)))
+__global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(YGeom<T, N, MODE>::WPE)))
 k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
        const cx<T> *__restrict__ qk, const double *__restrict__ dmx2) {
     using Y = YGeom<T, N, MODE>;
@@ -588,18 +641,28 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int nct = (Lx + TX - 1) / TX;
     const int ntiles = nct * nplanes;  // every plane incl. the z ghost planes
-    V2 pre[EPT];
+    constexpr int RPT = Y::RPT, NPRE = Y::RB ? RPT : EPT;
+    const int bg = tid / TP, bc2 = tid - bg * TP;  // row-blocked: group, column pair
+    V2 pre[NPRE];
     auto fetch = [&](int tile) {  // unconditional, clamped: see k_xfwd
         const int ct = tile % nct, pl = tile / nct;
         const size_t plane = (size_t)pl * Ly;
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            int e = tid + q * NT;
-            e = e < TP * NR ? e : TP * NR - 1;
-            const int r = e / TP, c2 = e - r * TP;
-            int col = ct * TX + 2 * c2;
+        if constexpr (Y::RB) {
+            int col = ct * TX + 2 * bc2;
             col = col < Lx ? col : Lx - 2;  // Lx is even
-            pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[(plane + R0 + r) * Lxs + col]));
+            const T *src = &W[(plane + R0 + (size_t)RPT * bg) * Lxs + col];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) pre[q] = ld_stream(reinterpret_cast<const V2 *>(src + (size_t)q * Lxs));
+        } else {
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                int e = tid + q * NT;
+                e = e < TP * NR ? e : TP * NR - 1;
+                const int r = e / TP, c2 = e - r * TP;
+                int col = ct * TX + 2 * c2;
+                col = col < Lx ? col : Lx - 2;  // Lx is even
+                pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[(plane + R0 + r) * Lxs + col]));
+            }
         }
     };
     int tile = blockIdx.x;
@@ -622,17 +685,27 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
     const int c0 = (tile % nct) * TX;
     const size_t plane = (size_t)(tile / nct) * Ly;
     __syncthreads();  // the previous tile's LDS lines have been stored
+    if constexpr (Y::RB) {
+        // natural order: rows RPT g + 2k, +1 are the components of slot (RPT/2) g + k
+        const LLine<T> c0l = L::line(lds, 2 * bc2), c1l = L::line(lds, 2 * bc2 + 1);
 #pragma unroll
-    for (int q = 0; q < EPT; ++q) {
-        const int e = tid + q * NT;
-        if (e < TP * NR) {
-            const int r = e / TP, c2 = e - r * TP;
-            const V2 v = pre[q];
-            if (MODE == 1) {  // a complex column, conjugated (inverse via the forward FFT)
-                L::line(lds, c2)[r] = cx<T>{v.x, -v.y};
-            } else {
-                L::real(lds, 2 * c2)[r] = v.x;
-                L::real(lds, 2 * c2 + 1)[r] = v.y;
+        for (int k = 0; k < RPT / 2; ++k) {
+            c0l[(RPT / 2) * bg + k] = cx<T>{pre[2 * k].x, pre[2 * k + 1].x};
+            c1l[(RPT / 2) * bg + k] = cx<T>{pre[2 * k].y, pre[2 * k + 1].y};
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < EPT; ++q) {
+            const int e = tid + q * NT;
+            if (e < TP * NR) {
+                const int r = e / TP, c2 = e - r * TP;
+                const V2 v = pre[q];
+                if (MODE == 1) {  // a complex column, conjugated (inverse via the forward FFT)
+                    L::line(lds, c2)[r] = cx<T>{v.x, -v.y};
+                } else {
+                    L::real(lds, 2 * c2)[r] = v.x;
+                    L::real(lds, 2 * c2 + 1)[r] = v.y;
+                }
             }
         }
     }
@@ -645,6 +718,31 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
         if (MODE == 2) c2r_pre<T, N, TPL>(my, t, tab + M);
         fft_forward<T, M, TPL>(my, t, tab);
     }
+    if constexpr (Y::RB) {
+        // Makhoul order (as in k_yfwd), the signs of the DCT-III's conjugate
+        // trick, and the scale; physical row n goes to W row n + 1
+        const int col = c0 + 2 * bc2;
+        if (col < Lx) {
+            const LLine<T> c0l = L::line(lds, 2 * bc2), c1l = L::line(lds, 2 * bc2 + 1);
+            V2 v[RPT];
+#pragma unroll
+            for (int j = 0; j < RPT / 4; ++j) {
+                const int ce = (RPT / 4) * bg + j, co = N / 2 - 1 - (RPT / 4) * bg - j;
+                const cx<T> a0 = c0l[ce], a1 = c1l[ce], b0 = c0l[co], b1 = c1l[co];
+                v[4 * j] = V2{a0.x, a1.x} * scale;
+                v[4 * j + 2] = V2{a0.y, a1.y} * -scale;
+                v[4 * j + 3] = V2{b0.x, b1.x} * scale;
+                v[4 * j + 1] = V2{b0.y, b1.y} * -scale;
+            }
+            T *dst = &W[(plane + 1 + (size_t)RPT * bg) * Lxs + col];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) st_stream(reinterpret_cast<V2 *>(dst + (size_t)q * Lxs), v[q]);
+            // wall ghost rows (fft_3d_poisson.f90:809-852)
+            if (bg == 0) st_stream(reinterpret_cast<V2 *>(dst - Lxs), bca0 * v[0] - bcb0 * v[1]);
+            if (bg == Y::G - 1)
+                st_stream(reinterpret_cast<V2 *>(dst + (size_t)RPT * Lxs), bca1 * v[RPT - 1] - bcb1 * v[RPT - 2]);
+        }
+    } else
     for (int e = tid; e < TP * (N + 2); e += NT) {
         const int rp = e / TP, c2 = e - rp * TP;
         const int col = c0 + 2 * c2;
