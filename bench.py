@@ -54,10 +54,18 @@ def cpu_baseline(n: int, nz_sample: int, reps: int = 5):
         ts.append(time.perf_counter() - t0)
     t = float(np.median(ts))
     npts = n * n * nz_sample
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cpu = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), cpu)
+    except OSError:
+        pass
     return {
         "value": round(npts / t / 1e6, 3),
         "unit": "Mpts/s",
         "cores": O.num_threads(),
+        "cpu_model": cpu,
+        "host_cpus": os.cpu_count(),
         "kind": "port",
         "sample": f"FP64 oracle (C restatement of the reference solve, OpenMP) on a {n}x{n}x{nz_sample} slab "
                   f"of the same workload, median of {reps} after 1 warm-up ({t:.3f} s/solve)",

@@ -496,7 +496,7 @@ int ppf_plan_create(const ppf_desc *desc, ppf_plan **out) {
     // slab pipeline for fields large enough to keep every stream busy
     // (PPF_SLABS overrides; 1 disables)
     {
-        // measured at 512^3 on MI355X: 1 slab 2.36 ms, 4 slabs 2.46, 8 slabs 2.47
+        // measured at 512^3 on MI355X: 1 slab 2.29 ms, 4 slabs 2.39, 8 slabs 2.40, 16 slabs 2.53
         int ns = 1;
         if (const char *e = std::getenv("PPF_SLABS")) ns = std::atoi(e);
         ns = std::max(1, std::min(ns, std::min(kMaxSlabs, d.n3m / 2)));
