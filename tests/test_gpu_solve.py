@@ -35,7 +35,8 @@ BCS = [(False, False), (True, True), (True, False), (False, True)]
 
 
 @pytest.mark.parametrize("pbc", BCS, ids=["NN", "PP", "PN", "NP"])
-@pytest.mark.parametrize("n1,n2,n3", [(8, 8, 8), (16, 12, 10), (32, 48, 20), (64, 64, 64), (96, 24, 6), (12, 192, 34)])
+@pytest.mark.parametrize("n1,n2,n3", [(8, 8, 8), (16, 12, 10), (32, 48, 20), (64, 64, 64), (96, 24, 6), (12, 192, 34),
+                                      (8, 1024, 6), (16, 768, 4)])
 def test_solve_matches_oracle(pbc, n1, n2, n3):
     ax, ay, az, f = _problem(n1, n2, n3, pbc, False, 1.5, n1 * 7 + n2 * 3 + n3)
     ref = O.solve(f, (ax.L, ay.L, az.L), (pbc[0], pbc[1], False), ax.dmx, ay.dmx, az.dmx, az.dx)
