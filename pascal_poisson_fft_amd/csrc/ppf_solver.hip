@@ -354,7 +354,6 @@ static int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hi
         case 12: part(integral_constant<int, 24>{}, integral_constant<int, 16>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
         case 13: part(integral_constant<int, 24>{}, integral_constant<int, 8>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
         case 14: part(integral_constant<int, 48>{}, integral_constant<int, 8>{}, integral_constant<int, 512>{}, integral_constant<int, 1>{}); break;
-        case 15: part(integral_constant<int, 32>{}, integral_constant<int, 32>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
         default: break;
     }
     if (!done) {
@@ -751,7 +750,7 @@ int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1
 }
 
 int ppf_plan_set_zvariant(ppf_plan *p, int zvariant) {
-    if (!p || zvariant < 0 || zvariant > 15) return PPF_ERR_INVALID_ARG;
+    if (!p || zvariant < 0 || zvariant > 14) return PPF_ERR_INVALID_ARG;
     p->zvariant = zvariant;
     return PPF_OK;
 }
