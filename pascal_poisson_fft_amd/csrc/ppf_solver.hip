@@ -671,6 +671,10 @@ static int solve_fp32_ir(ppf_plan *p, const double *f, double *P, const double *
         }
         converged = true;
     } else {
+        // The residual form of pass 1 computes ||r|| and transforms r for the
+        // next sweep; after a sweep the plain residual kernel (about half its
+        // time) checks convergence, so a converged solve does not pay for a
+        // transform it discards.  Both evaluate the same FP64 operator.
         for (;;) {
             // residual of the current iterate, transformed by pass 1
             unsigned g = 0;
@@ -688,6 +692,9 @@ static int solve_fp32_ir(ppf_plan *p, const double *f, double *P, const double *
                 return rc;
             PPF_HIP(hipGetLastError());
             ++sweeps;
+            if ((rc = ppf_residual(p, f, P, dmx3, dx3, s, &rel))) return rc;
+            if (rel <= tol) { converged = true; break; }
+            if (sweeps == maxs || !(rel == rel)) break;
         }
         if (sweeps > 0 && (rc = refresh_ghosts(p, P, dmx1, dmx2, dmx3, s))) return rc;
     }
