@@ -924,14 +924,12 @@ k_zcoef(const double *__restrict__ dx3, const double *__restrict__ dmx3, int n3m
 
 // WPE: minimum waves per SIMD the register allocation must allow (occupancy
 // is what lets one workgroup's loads overlap another's arithmetic here).
-// CT: arithmetic type (FP64, or FP32 for the FP32 storage of refinement
-// plans); the workgroup of tile tile_base >= 0 alone, else blockIdx.x with
-// tile tile_skip left out (the FP32 launch leaves the (0,0) mode's tile to
-// an FP64 launch).
+// CT: arithmetic type (FP64 for every plan: FP32 arithmetic on FP32 storage
+// measured no faster and costs the refinement a sweep).
 template <typename T, int K, int S, int NTMAX, int WPE, typename CT = double>
 __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double *__restrict__ zap,
-        double *__restrict__ mean_out, int P, int tile_base = -1, int tile_skip = 1 << 30) {
+        double *__restrict__ mean_out, int P) {
     extern __shared__ double zsh[];
     const int n3m = z.n3m;
     const int NP = P * K;           // padded rows
@@ -944,7 +942,7 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     CT *ps = ra;                    // [P][S]: partial sums, once the reduction is done
     const int tid = threadIdx.x, s = tid % S, p = tid / S;
     const int tiles = (z.Lx + S - 1) / S;
-    const int blk = tile_base >= 0 ? tile_base : (int)blockIdx.x + ((int)blockIdx.x >= tile_skip ? 1 : 0);
+    const int blk = (int)blockIdx.x;
     const int jp = z.jlo + blk / tiles;
     const int kx = (blk % tiles) * S + s;
     const bool active = kx < z.Lx;
@@ -1393,11 +1391,15 @@ k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m) {
         const int k0 = kb * KZ;
         const long pc0 = ((long)(k0 + 1) * L2 + (j + 1)) * L1 + (i + 1);
         double zm = P[pc0 - pl], c = P[pc0];  // ghost plane below the first (finite, weight 0 at the wall)
+        // planes past n3m load the last plane again (clamped addresses, no
+        // branch), so all KZ planes' loads issue together; their terms are
+        // masked out of the sums
 #pragma unroll
         for (int kk = 0; kk < KZ; ++kk) {
-            const int k = k0 + kk;
-            if (k >= n3m) break;
-            const long pc = pc0 + (long)kk * pl;
+            const int k0k = k0 + kk;
+            const bool in = k0k < n3m;
+            const int k = in ? k0k : n3m - 1;
+            const long pc = pc0 + (long)(k - k0) * pl;
             const double zp = P[pc + pl];
             const double xm = P[pc + dxm], xp = P[pc + dxp], ym = P[pc + dym], yp = P[pc + dyp];
             const double fv = ld_stream(&f[((size_t)k * n2m + j) * n1m + i]);
@@ -1405,9 +1407,11 @@ k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m) {
             const double lp = (xp - 2.0 * c + xm) * xr.ih1 + (yp - 2.0 * c + ym) * xr.ih2 + am * zm - (am + ap) * c +
                               ap * zp;
             const double r = fv - lp;
-            sr += r * r;
-            sf += fv * fv;
-            sx += c;
+            if (in) {
+                sr += r * r;
+                sf += fv * fv;
+                sx += c;
+            }
             zm = c;
             c = zp;
         }

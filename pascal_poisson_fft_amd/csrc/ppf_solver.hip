@@ -316,7 +316,7 @@ static int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hi
         // FP64 arithmetic for FP32 storage too: FP32 arithmetic measured no
         // faster at 512^3 (0.509 vs 0.515 ms) and costs the refinement a sweep
         hipLaunchKernelGGL((k_zpart<T, K, S, NTMAX, WPE>), dim3((unsigned)ntiles), dim3(S * P), shm, s, W, z,
-                           p->zcoef, p->zcoef + n3m, p->mean, P, -1, 1 << 30);
+                           p->zcoef, p->zcoef + n3m, p->mean, P);
         done = true;
     };
     // pipelined variant: persistent, prefetching, wavefront cyclic reduction
@@ -796,9 +796,15 @@ int ppf_residual(ppf_plan *p, const double *f, const double *P, const double *dm
     const int n3m = p->d.n3m;
     hipLaunchKernelGGL(k_zcoef, dim3((n3m + 255) / 256), dim3(256), 0, s, dx3, dmx3, n3m, p->zcoef, p->zcoef + n3m);
     const XRes xr = make_xres(p, P);
-    constexpr int KZ = 8;
+#ifndef PPF_RES_KZ
+#define PPF_RES_KZ 32
+#endif
+#ifndef PPF_RES_WGS
+#define PPF_RES_WGS 8
+#endif
+    constexpr int KZ = PPF_RES_KZ;  // planes per tile
     const long ntiles = (long)((p->d.n1m + 255) / 256) * p->d.n2m * ((n3m + KZ - 1) / KZ);
-    const int g = (int)std::min<long>(std::min<long>(p->npartial, 8L * num_cus()), ntiles);
+    const int g = (int)std::min<long>(std::min<long>(p->npartial, (long)PPF_RES_WGS * num_cus()), ntiles);
     hipLaunchKernelGGL(k_residual<KZ>, dim3(g), dim3(256), 0, s, f, xr, p->d.n1m, p->d.n2m, n3m);
     hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, s, p->partial, g, p->red);
     PPF_HIP(hipGetLastError());
