@@ -488,15 +488,18 @@ struct YGeom {
     static_assert(NT <= kMaxNT, "y-pass workgroup exceeds kMaxNT threads");
     static_assert(lds_bytes(TX) <= 160 * 1024, "y-pass tile exceeds the CU's LDS");
     using L = LdsLines<T, M, S, NL>;
-    // Row-blocked mapping (DCT columns): thread (g, c2) owns the RPT
-    // consecutive rows RPT*g .. RPT*g+RPT-1 of column pair c2.  The Makhoul
-    // order then puts a thread's rows into 2*(RPT/4) complex LDS slots per
-    // column, known once per thread, and LDS is read and written as whole
-    // complex values (16 bytes in FP64).
+    // Row-blocked mapping (real columns, DCT or R2C): thread (g, c2) owns the
+    // RPT consecutive rows RPT*g .. RPT*g+RPT-1 of column pair c2.  In the
+    // DCT's Makhoul order a thread's rows fill 2*(RPT/4) complex LDS slots
+    // per column, in natural order RPT/2 slots; either way the slots are
+    // known once per thread and LDS is read and written as whole complex
+    // values (16 bytes in FP64).  The R2C spectrum's last slot (rows N, N+1)
+    // belongs to the last group.
     static constexpr int TP = TX / 2;
     static constexpr int G = NT / TP;
     static constexpr int RPT = G > 0 ? N / G : 0;
-    static constexpr bool RB = MODE == 0 && NT % TP == 0 && G * RPT == N && RPT % 4 == 0;
+    static constexpr bool RB = NT % TP == 0 && G * RPT == N &&
+                               ((MODE == 0 && RPT % 4 == 0) || (MODE == 2 && RPT % 2 == 0));
     // registers: uncapped (PPF_YWPE = 1).  The row-blocked 512-point FP64
     // kernels hold 178-186 VGPRs; a cap of 128 (two 512-thread workgroups per
     // CU) spills about 50 of them and measured 1.7x slower.
@@ -549,7 +552,7 @@ k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
     const int c0 = (tile % nct) * TX;
     const size_t plane = (size_t)(tile / nct + 1) * Ly;
     __syncthreads();  // the previous tile's LDS lines have been stored
-    if constexpr (Y::RB) {
+    if constexpr (Y::RB && MODE == 0) {
         // Makhoul order: row RPT g + 4j (+2) is component 0 (1) of complex slot
         // (RPT/4) g + j; row RPT g + 4j + 3 (+1) is component 0 (1) of slot
         // N/2 - 1 - (RPT/4) g - j
@@ -561,6 +564,14 @@ k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
             c1l[ce] = cx<T>{pre[4 * j].y, pre[4 * j + 2].y};
             c0l[co] = cx<T>{pre[4 * j + 3].x, pre[4 * j + 1].x};
             c1l[co] = cx<T>{pre[4 * j + 3].y, pre[4 * j + 1].y};
+        }
+    } else if constexpr (Y::RB) {
+        // natural order: rows RPT g + 2k, +1 are the components of slot (RPT/2) g + k
+        const LLine<T> c0l = L::line(lds, 2 * bc2), c1l = L::line(lds, 2 * bc2 + 1);
+#pragma unroll
+        for (int k = 0; k < RPT / 2; ++k) {
+            c0l[(RPT / 2) * bg + k] = cx<T>{pre[2 * k].x, pre[2 * k + 1].x};
+            c1l[(RPT / 2) * bg + k] = cx<T>{pre[2 * k].y, pre[2 * k + 1].y};
         }
     } else {
 #pragma unroll
@@ -604,6 +615,11 @@ k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
                 st_stream(reinterpret_cast<V2 *>(dst + (size_t)(2 * k) * Lxs), V2{z0.x, z1.x});
                 st_stream(reinterpret_cast<V2 *>(dst + (size_t)(2 * k + 1) * Lxs), V2{z0.y, z1.y});
             }
+            if (MODE == 2 && bg == Y::G - 1) {  // R2C: spectral rows N, N+1 (slot N/2)
+                const cx<T> z0 = c0l[N / 2], z1 = c1l[N / 2];
+                st_stream(reinterpret_cast<V2 *>(dst + (size_t)RPT * Lxs), V2{z0.x, z1.x});
+                st_stream(reinterpret_cast<V2 *>(dst + (size_t)(RPT + 1) * Lxs), V2{z0.y, z1.y});
+            }
         }
     } else {
         for (int e = tid; e < TP * NR; e += NT) {
@@ -644,6 +660,7 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
     constexpr int RPT = Y::RPT, NPRE = Y::RB ? RPT : EPT;
     const int bg = tid / TP, bc2 = tid - bg * TP;  // row-blocked: group, column pair
     V2 pre[NPRE];
+    V2 pre_x[2] = {V2{0, 0}, V2{0, 0}};  // row-blocked R2C: rows N, N+1 (last group)
     auto fetch = [&](int tile) {  // unconditional, clamped: see k_xfwd
         const int ct = tile % nct, pl = tile / nct;
         const size_t plane = (size_t)pl * Ly;
@@ -653,6 +670,10 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
             const T *src = &W[(plane + R0 + (size_t)RPT * bg) * Lxs + col];
 #pragma unroll
             for (int q = 0; q < RPT; ++q) pre[q] = ld_stream(reinterpret_cast<const V2 *>(src + (size_t)q * Lxs));
+            if (MODE == 2 && bg == Y::G - 1) {  // R2C: spectral rows N, N+1
+                pre_x[0] = ld_stream(reinterpret_cast<const V2 *>(src + (size_t)RPT * Lxs));
+                pre_x[1] = ld_stream(reinterpret_cast<const V2 *>(src + (size_t)(RPT + 1) * Lxs));
+            }
         } else {
 #pragma unroll
             for (int q = 0; q < EPT; ++q) {
@@ -693,6 +714,10 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
             c0l[(RPT / 2) * bg + k] = cx<T>{pre[2 * k].x, pre[2 * k + 1].x};
             c1l[(RPT / 2) * bg + k] = cx<T>{pre[2 * k].y, pre[2 * k + 1].y};
         }
+        if (MODE == 2 && bg == Y::G - 1) {
+            c0l[N / 2] = cx<T>{pre_x[0].x, pre_x[1].x};
+            c1l[N / 2] = cx<T>{pre_x[0].y, pre_x[1].y};
+        }
     } else {
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
@@ -718,7 +743,26 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
         if (MODE == 2) c2r_pre<T, N, TPL>(my, t, tab + M);
         fft_forward<T, M, TPL>(my, t, tab);
     }
-    if constexpr (Y::RB) {
+    if constexpr (Y::RB && MODE == 2) {
+        // natural order; odd samples change sign (the C2R's conjugate trick);
+        // periodic ghost rows 0 <- sample N-1 (last group), N+1 <- sample 0
+        const int col = c0 + 2 * bc2;
+        if (col < Lx) {
+            const LLine<T> c0l = L::line(lds, 2 * bc2), c1l = L::line(lds, 2 * bc2 + 1);
+            V2 v[RPT];
+#pragma unroll
+            for (int k = 0; k < RPT / 2; ++k) {
+                const cx<T> a0 = c0l[(RPT / 2) * bg + k], a1 = c1l[(RPT / 2) * bg + k];
+                v[2 * k] = V2{a0.x, a1.x} * scale;
+                v[2 * k + 1] = V2{a0.y, a1.y} * -scale;
+            }
+            T *dst = &W[(plane + 1 + (size_t)RPT * bg) * Lxs + col];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) st_stream(reinterpret_cast<V2 *>(dst + (size_t)q * Lxs), v[q]);
+            if (bg == 0) st_stream(reinterpret_cast<V2 *>(&W[(plane + N + 1) * Lxs + col]), v[0]);
+            if (bg == Y::G - 1) st_stream(reinterpret_cast<V2 *>(&W[plane * Lxs + col]), v[RPT - 1]);
+        }
+    } else if constexpr (Y::RB) {
         // Makhoul order (as in k_yfwd), the signs of the DCT-III's conjugate
         // trick, and the scale; physical row n goes to W row n + 1
         const int col = c0 + 2 * bc2;
