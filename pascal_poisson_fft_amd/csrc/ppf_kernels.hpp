@@ -224,7 +224,14 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 // r = f - L x (in = f, xr.P = x) rather than in itself, and the workgroup's
 // sums of r^2, f^2 and x go to xr.partial.
 template <typename T, int N, bool PER, bool RES = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RES ? 1 : PPF_XWPE)))
+// The refinement forms carry the residual stencil / the prefetched iterate:
+// uncapped they take one wave per SIMD (256 VGPRs + AGPRs); a cap of two
+// spills 124 and 24 bytes per lane and measured 0.36 ms faster per 512^3
+// refinement solve.
+#ifndef PPF_XRES_WPE
+#define PPF_XRES_WPE 2
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RES ? PPF_XRES_WPE : PPF_XWPE)))
 k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, int Ly,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk, XRes xr,
        ZRows zr) {
@@ -266,6 +273,9 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
                     sf += fv0 * fv0 + fv1 * fv1;
                     sx += xv0 + xv1;
                 }
+                // one pair's stencil loads at a time: r is needed at once, so
+                // keeping every pair's loads in flight only costs registers
+                __builtin_amdgcn_sched_barrier(0);
             } else {
                 pre[q] = ld_stream(reinterpret_cast<const D2 *>(&in[(size_t)line * N + n]));
             }
@@ -334,7 +344,10 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
 // prefetched with the W rows.
 // ---------------------------------------------------------------------------
 template <typename T, int N, bool PER, bool ACC = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ACC ? 1 : PPF_XWPE)))
+#ifndef PPF_XACC_WPE
+#define PPF_XACC_WPE 2
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ACC ? PPF_XACC_WPE : PPF_XWPE)))
 k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk,
        const double *__restrict__ dmx1, const double *__restrict__ mean, const double *__restrict__ xsum,
