@@ -225,13 +225,18 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 // sums of r^2, f^2 and x go to xr.partial.
 template <typename T, int N, bool PER, bool RES = false>
 // The refinement forms carry the residual stencil / the prefetched iterate:
-// uncapped they take one wave per SIMD (256 VGPRs + AGPRs); a cap of two
-// spills 124 and 24 bytes per lane and measured 0.36 ms faster per 512^3
-// refinement solve.
+// uncapped they take one wave per SIMD (256 VGPRs + AGPRs).  From lines of
+// PPF_XIR_CAP_N points on they are capped at two waves: that spills 124 and
+// 24 bytes per lane and measured 0.36 ms faster per 512^3 refinement solve,
+// while on the 32..256-point lines of the 96-problem batch the spills cost
+// more than the occupancy gains (FP32+IR batch 12.3 vs 11.3 ms).
 #ifndef PPF_XRES_WPE
 #define PPF_XRES_WPE 2
 #endif
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RES ? PPF_XRES_WPE : PPF_XWPE)))
+#ifndef PPF_XIR_CAP_N
+#define PPF_XIR_CAP_N 512
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RES ? (N >= PPF_XIR_CAP_N ? PPF_XRES_WPE : 1) : PPF_XWPE)))
 k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, int Ly,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk, XRes xr,
        ZRows zr) {
@@ -347,7 +352,7 @@ template <typename T, int N, bool PER, bool ACC = false>
 #ifndef PPF_XACC_WPE
 #define PPF_XACC_WPE 2
 #endif
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ACC ? PPF_XACC_WPE : PPF_XWPE)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ACC ? (N >= PPF_XIR_CAP_N ? PPF_XACC_WPE : 1) : PPF_XWPE)))
 k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk,
        const double *__restrict__ dmx1, const double *__restrict__ mean, const double *__restrict__ xsum,
