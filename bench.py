@@ -34,9 +34,10 @@ BYTES_PER_PT_STAGE = 16.0  # one FP64 read + one FP64 write per cell per pass
 BYTES_PER_PT_SOLVE = 80.0  # five passes
 
 
-def cpu_baseline(n: int, nz_sample: int, reps: int = 5):
+def cpu_baseline(n: int, nz_sample: int, reps: int = 20):
     """FP64 CPU oracle (the project's restatement of the reference method) on an
-    n x n x nz_sample slab of the same workload; median of `reps` after 1 warm-up."""
+    n x n x nz_sample slab of the same workload; median of `reps` (at most about
+    20 s of CPU work) after 1 warm-up."""
     from oracle import oracle as O
     from pascal_poisson_fft_amd import workloads as W
 
