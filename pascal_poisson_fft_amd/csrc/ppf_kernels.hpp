@@ -111,6 +111,11 @@ struct XGeom {
     static constexpr int LPB = (256 / TPL) > 0 ? (256 / TPL) : 1;
     static constexpr int NT = LPB * TPL;
     using L = LdsLines<T, G::M, G::STRIDE, LPB>;
+    // register target: PPF_XWPE waves per SIMD, or fewer where the LDS tile
+    // (very short or 1024-point lines) admits fewer workgroups per CU
+    static constexpr int WG_LDS = (160 * 1024) / (L::SIZE * (int)sizeof(T));
+    static constexpr int WPE_LDS = (WG_LDS * (NT / 64)) / 4 > 0 ? (WG_LDS * (NT / 64)) / 4 : 1;
+    static constexpr int WPE = PPF_XWPE < WPE_LDS ? PPF_XWPE : WPE_LDS;
 };
 
 // q = a / b for 0 <= a < 2^24 via a float reciprocal and one correction
@@ -236,7 +241,7 @@ template <typename T, int N, bool PER, bool RES = false>
 #ifndef PPF_XIR_CAP_N
 #define PPF_XIR_CAP_N 512
 #endif
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RES ? (N >= PPF_XIR_CAP_N ? PPF_XRES_WPE : 1) : PPF_XWPE)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RES ? (N >= PPF_XIR_CAP_N ? PPF_XRES_WPE : 1) : XGeom<T, N, PER>::WPE)))
 k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, int Ly,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk, XRes xr,
        ZRows zr) {
@@ -352,7 +357,7 @@ template <typename T, int N, bool PER, bool ACC = false>
 #ifndef PPF_XACC_WPE
 #define PPF_XACC_WPE 2
 #endif
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ACC ? (N >= PPF_XIR_CAP_N ? PPF_XACC_WPE : 1) : PPF_XWPE)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ACC ? (N >= PPF_XIR_CAP_N ? PPF_XACC_WPE : 1) : XGeom<T, N, PER>::WPE)))
 k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk,
        const double *__restrict__ dmx1, const double *__restrict__ mean, const double *__restrict__ xsum,

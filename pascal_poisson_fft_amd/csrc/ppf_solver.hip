@@ -845,9 +845,12 @@ int ppf_copy_timed(double *dst, const double *src, size_t count, int reps, void 
     hipStream_t s = static_cast<hipStream_t>(stream);
     const size_t n2 = count / 2;
     const unsigned grid = 8u * (unsigned)num_cus();
-    hipEvent_t e0, e1;
-    PPF_HIP(hipEventCreate(&e0));
-    PPF_HIP(hipEventCreate(&e1));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess) return PPF_ERR_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return PPF_ERR_HIP;
+    }
     int rc = PPF_OK;
     if (hipEventRecord(e0, s) != hipSuccess) rc = PPF_ERR_HIP;
     for (int r = 0; r < reps && rc == PPF_OK; ++r) {
@@ -859,8 +862,8 @@ int ppf_copy_timed(double *dst, const double *src, size_t count, int reps, void 
                          hipEventElapsedTime(ms, e0, e1) != hipSuccess))
         rc = PPF_ERR_HIP;
     if (rc == PPF_OK) *ms /= (float)reps;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return rc;
 }
 
