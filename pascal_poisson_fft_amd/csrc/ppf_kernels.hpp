@@ -517,7 +517,9 @@ struct YGeom {
     // per column, in natural order RPT/2 slots; either way the slots are
     // known once per thread and LDS is read and written as whole complex
     // values (16 bytes in FP64).  The R2C spectrum's last slot (rows N, N+1)
-    // belongs to the last group.
+    // belongs to the last group.  Complex (C2C) columns keep the
+    // element-strided mapping: row-blocked, their inverse pass measured
+    // 0.427 ms against 0.385 ms at 512^3.
     static constexpr int TP = TX / 2;
     static constexpr int G = NT / TP;
     static constexpr int RPT = G > 0 ? N / G : 0;
