@@ -321,17 +321,24 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
             r2c_post<T, N, TPL>(my, t, tab + M);
         else
             dct_post<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
-        for (int e = tid; e < LPB * HX; e += NT) {
-            const int li = e / HX, h = e - li * HX;
+        // W row of prhs line (j,k): (k+1)*Ly + j + 1 = line + n2m + 3 + 2k.
+        // The R2C row's Nyquist pair (h = N/2) is stored apart, so the main
+        // loop covers N/2 pairs per line (no ragged last round).
+        constexpr int HM = PER ? HX - 1 : HX;
+        auto store_pair = [&](int li, int h) {
             const int line = tile * LPB + li;
             if (line < nlines) {
-                // W row of prhs line (j,k): (k+1)*Ly + j + 1 = line + n2m + 3 + 2k
                 const int k = fdiv_small(line, n2m, rn2m);
                 const cx<T> z = L::line(lds, li)[h];
                 st_stream(reinterpret_cast<V2 *>(&W[((size_t)line + (size_t)(Ly + 1) + 2 * (size_t)k) * LXS + 2 * h]),
                           V2{z.x, z.y});
             }
+        };
+        for (int e = tid; e < LPB * HM; e += NT) {
+            const int li = e / HM;
+            store_pair(li, e - li * HM);
         }
+        if (PER && tid < LPB) store_pair(tid, HX - 1);
     }
     if constexpr (RES) {
         __shared__ double red[NT];
