@@ -1009,10 +1009,12 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     extern __shared__ double zsh[];
     const int n3m = z.n3m;
     const int NP = P * K;           // padded rows
-    double *am = zsh;               // [NP], 0 past n3m
-    double *ap = am + NP;           // [NP], 0 past n3m
+    constexpr int KS = (K + S - 1) / S;           // table rows per thread
+    const int NPAD = KS * (int)blockDim.x;        // >= NP: every thread writes KS entries
+    double *am = zsh;               // [NPAD], 0 past n3m
+    double *ap = am + NPAD;         // [NPAD], 0 past n3m
     const int PS = P * S;
-    CT *ra = reinterpret_cast<CT *>(ap + NP);  // [3][P][S]: first-row equations, then reduction buffer
+    CT *ra = reinterpret_cast<CT *>(ap + NPAD);  // [3][P][S]: first-row equations, then reduction buffer
     CT *rc = ra + 3 * PS;           // [3][P][S]: reduction buffer
     CT *xe = rc + 3 * PS;           // [P][S]: partition end values x_{K-1}
     CT *ps = ra;                    // [P][S]: partial sums, once the reduction is done
@@ -1030,26 +1032,47 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     const int g0 = p * K;
     // loads are clamped in bounds and selected, so the sweep code is branch-free
     const int glast = n3m - 1;
-    // all loads first (clamped addresses, no consumer), masks afterwards
+    // Row coefficients (the per-solve table filled by pass 1) are loaded
+    // first: loads complete in order, so waiting for them does not wait for
+    // the line loads issued after them.  Their LDS writes are unconditional
+    // (the arrays are padded to KS entries per thread), which keeps the
+    // loads ahead of the line loads.
+    double ca[KS], cp[KS];
+#pragma unroll
+    for (int j = 0; j < KS; ++j) {
+        const int k = tid + j * (int)blockDim.x, kc = k < glast ? k : glast;
+        ca[j] = zam[kc];
+        cp[j] = zap[kc];
+    }
+    double bcd[4];  // z wall extrapolation, from the same table
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bcd[q] = zap[n3m + q];
+    // line loads in the order the two sweeps below consume them (rows 1 and
+    // K-2 first, then inwards; rows 0 and K-1 last), so the sweeps start as
+    // soon as the first rows arrive; nothing consumes them before that
     CT d[K];
 #pragma unroll
-    for (int l = 0; l < K; ++l) {
+    for (int q = 0; q < K; ++q) {
+        const int l = q < K - 2 ? ((q & 1) ? K - 2 - q / 2 : 1 + q / 2) : (q == K - 2 ? 0 : K - 1);
         const int g = g0 + l;
         d[l] = (CT)ld_stream(&w[(size_t)((g < glast ? g : glast) + 1) * pstride]);
     }
 #pragma unroll
-    for (int l = 0; l < K; ++l) d[l] = (active && g0 + l < n3m) ? d[l] : CT(0);
-    if (pin && g0 == 0) d[0] = CT(0);
-    // row coefficients (the per-solve table filled by pass 1) go to LDS while
-    // the line loads above are still in flight
-    for (int k = tid; k < NP; k += blockDim.x) {
-        am[k] = k < n3m ? zam[k] : 0.0;
-        ap[k] = k < n3m ? zap[k] : 0.0;
+    for (int j = 0; j < KS; ++j) {
+        const int k = tid + j * (int)blockDim.x;
+        am[k] = k < n3m ? ca[j] : 0.0;
+        ap[k] = k < n3m ? cp[j] : 0.0;
     }
-    CT bc[4];  // z wall extrapolation, from the same table
+    CT bc[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) bc[q] = (CT)zap[n3m + q];
+    for (int q = 0; q < 4; ++q) bc[q] = (CT)bcd[q];
     __syncthreads();
+    // a row's value is masked where it is first used: zero past n3m, in
+    // inactive columns, and in the pinned first row of the (0,0) mode
+    auto dmask = [&](int l) {
+        const int g = g0 + l;
+        d[l] = (active && g < n3m && !(pin && g == 0)) ? d[l] : CT(0);
+    };
     // identity rows: padding past n3m, and the pinned first row of the (0,0) mode
     auto coef = [&](int l, CT &a, CT &b, CT &c) {
         const int g = g0 + l;
@@ -1066,6 +1089,8 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     CT ah, ch, dh, Af, Df, Cf;
     {
         CT a, b, c;
+        dmask(K - 2);
+        dmask(1);
         coef(K - 2, a, b, c);
         double r = frcp(b);
         ah = r * a; ch = r * c; dh = r * d[K - 2];
@@ -1077,6 +1102,8 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
         for (int i = 1; i <= K - 2; ++i) {
             const int lb = K - 2 - i;  // reverse row
             const int lf = 1 + i;      // forward row
+            dmask(lb);
+            dmask(lf);
             CT a1, b1, c1, a2, b2, c2;
             coef(lb, a1, b1, c1);
             coef(lf, a2, b2, c2);
