@@ -106,6 +106,24 @@ int ppf_plan_destroy(ppf_plan *plan);
 int ppf_solve_sweeps(ppf_plan *plan, const double *prhs, double *p, const double *dmx1, const double *dmx2,
                      const double *dmx3, const double *dx3, int sweeps, void *stream);
 
+/* A solve that never synchronises the host, for callers that capture it in a
+ * hipGraph or replay it on new right-hand sides.  FP32_IR plans: the FP32
+ * solve, min_sweeps corrections, then -- up to max_sweeps -- an FP64 residual
+ * test on the device before each further correction; once the relative
+ * residual is <= tol every later correction and test is a no-op (tol < 0:
+ * all max_sweeps corrections run).  Ghost layers are re-derived at the end.
+ * Only kernel launches on `stream` (no memset, no copy).  FP64 plans: a plain
+ * solve.  No counterpart in the reference (its solve is FP64 only and
+ * synchronous, fft_3d_poisson.f90:353-626); used by the batched solve. */
+int ppf_solve_async(ppf_plan *plan, const double *prhs, double *p, const double *dmx1, const double *dmx2,
+                    const double *dmx3, const double *dx3, int min_sweeps, void *stream);
+
+/* After a ppf_solve_async on `stream`: synchronises the stream and reports
+ * the corrections it ran and the last relative residual its device-side test
+ * measured.  Returns PPF_ERR_NOT_CONVERGED when that residual is above the
+ * plan's tol (tol >= 0).  FP64 plans: sweeps 0, rel_residual -1. */
+int ppf_solve_async_result(ppf_plan *plan, void *stream, ppf_stats *stats);
+
 /* Autotuner: times the z-pass kernel shapes (partitioned Thomas with K rows
  * per thread and S columns per workgroup, and the pipelined wavefront cyclic
  * reduction) on this plan's extents with full solves of the given data, keeps
@@ -140,14 +158,17 @@ typedef struct ppf_batch ppf_batch;
  * counterpart there.
  * The first ppf_batch_solve solves every problem directly (FP32_IR problems
  * with their convergence test), records the sweeps each needed, and captures
- * the whole batch -- every problem as its own branch, FP32_IR problems with
- * those sweep counts -- into one hipGraph; later calls replay the graph. */
+ * the whole batch -- every problem as its own branch, FP32_IR problems as
+ * ppf_solve_async with those sweep counts as min_sweeps -- into one hipGraph;
+ * later calls replay the graph. */
 int ppf_batch_create(int nprob, const ppf_desc *descs, const ppf_batch_io *io, int flags, ppf_batch **batch);
 
 /* Solves every problem of the batch on `stream`.  stats (nprob entries or
- * NULL) receives per-problem sweeps and, when given, the measured relative
- * residual (which synchronises).  Without stats a graph replay returns
- * without synchronising. */
+ * NULL) receives per-problem sweeps and the measured relative residual (which
+ * synchronises): for FP32_IR problems of a replay, the sweeps and residual of
+ * their device-side test, and PPF_ERR_NOT_CONVERGED is returned if any of
+ * them stayed above its tol.  Without stats a replay returns without
+ * synchronising. */
 int ppf_batch_solve(ppf_batch *batch, void *stream, ppf_stats *stats);
 
 int ppf_batch_destroy(ppf_batch *batch);

@@ -93,6 +93,8 @@ def lib() -> C.CDLL:
     L.ppf_set_device.argtypes = [I]
     L.ppf_copy_timed.argtypes = [vp, vp, sz, I, vp, C.POINTER(C.c_float)]
     L.ppf_solve_sweeps.argtypes = [vp, vp, vp, vp, vp, vp, vp, I, vp]
+    L.ppf_solve_async.argtypes = [vp, vp, vp, vp, vp, vp, vp, I, vp]
+    L.ppf_solve_async_result.argtypes = [vp, vp, C.POINTER(Stats)]
     L.ppf_plan_tune.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(I)]
     L.ppf_plan_set_zvariant.argtypes = [vp, I]
     L.ppf_batch_create.argtypes = [I, C.POINTER(Desc), C.POINTER(BatchIO), I, C.POINTER(vp)]
@@ -241,6 +243,20 @@ class FFTPoisson:
         """Exactly `sweeps` refinement corrections, no convergence test (capture-safe)."""
         _check(lib().ppf_solve_sweeps(self.handle, prhs.ptr, p.ptr, dmx1.ptr, dmx2.ptr, dmx3.ptr, dx3.ptr,
                                       int(sweeps), stream), "ppf_solve_sweeps")
+
+    def solve_async(self, prhs: DeviceArray, p: DeviceArray, dmx1: DeviceArray, dmx2: DeviceArray,
+                    dmx3: DeviceArray, dx3: DeviceArray, min_sweeps: int = 0, stream=None) -> None:
+        """Solve with the FP32_IR convergence test on the device (no host synchronisation)."""
+        _check(lib().ppf_solve_async(self.handle, prhs.ptr, p.ptr, dmx1.ptr, dmx2.ptr, dmx3.ptr, dx3.ptr,
+                                     int(min_sweeps), stream), "ppf_solve_async")
+
+    def async_result(self, stream=None, allow_not_converged: bool = False) -> Stats:
+        """Sweeps and residual of the last solve_async (synchronises)."""
+        st = Stats()
+        rc = lib().ppf_solve_async_result(self.handle, stream, C.byref(st))
+        if not (rc == PPF_ERR_NOT_CONVERGED and allow_not_converged):
+            _check(rc, "ppf_solve_async_result")
+        return st
 
     def tune(self, prhs: DeviceArray, p: DeviceArray, dmx1: DeviceArray, dmx2: DeviceArray, dmx3: DeviceArray,
              dx3: DeviceArray, stream=None) -> int:

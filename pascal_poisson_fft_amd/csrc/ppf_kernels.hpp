@@ -27,6 +27,11 @@ constexpr int kMaxNT = 512;
 // straddles two lines in every tile segment).
 __host__ __device__ constexpr int w_stride(int lx, int tbytes) { return (lx * tbytes + 127) / 128 * 128 / tbytes; }
 
+// Gate of the device-side refinement test: a pass launched with a gate runs
+// only while gate[0] is nonzero (more sweeps needed); nullptr = always.  The
+// test is uniform over the grid, so a gated kernel returns before any barrier.
+__device__ __forceinline__ bool gated_off(const int *gate) { return gate && *gate == 0; }
+
 // Streaming global accesses of the pass kernels: every element of a pass is
 // read once and written once, so PPF_NT_LOAD / PPF_NT_STORE (default on)
 // mark them nontemporal (no reuse expected in the caches).
@@ -244,7 +249,8 @@ template <typename T, int N, bool PER, bool RES = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RES ? (N >= PPF_XIR_CAP_N ? PPF_XRES_WPE : 1) : XGeom<T, N, PER>::WPE)))
 k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, int Ly,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk, XRes xr,
-       ZRows zr) {
+       ZRows zr, const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
     using X = XGeom<T, N, PER>;
     using G = typename X::G;
     using L = typename X::L;
@@ -368,7 +374,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ACC ? 
 k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk,
        const double *__restrict__ dmx1, const double *__restrict__ mean, const double *__restrict__ xsum,
-       double inv_cells) {
+       double inv_cells, const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
     using X = XGeom<T, N, PER>;
     using G = typename X::G;
     using L = typename X::L;
@@ -541,7 +548,8 @@ struct YGeom {
 template <typename T, int N, int MODE>
 __global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(YGeom<T, N, MODE>::WPE)))
 k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
-       const cx<T> *__restrict__ qk) {
+       const cx<T> *__restrict__ qk, const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
     using Y = YGeom<T, N, MODE>;
     using L = typename Y::L;
     constexpr int M = Y::M, TPL = Y::TPL, TX = Y::TX, NT = Y::NT;
@@ -676,7 +684,8 @@ k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
 template <typename T, int N, int MODE>
 __global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(YGeom<T, N, MODE>::WPE)))
 k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
-       const cx<T> *__restrict__ qk, const double *__restrict__ dmx2) {
+       const cx<T> *__restrict__ qk, const double *__restrict__ dmx2, const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
     using Y = YGeom<T, N, MODE>;
     using L = typename Y::L;
     constexpr int M = Y::M, TPL = Y::TPL, TX = Y::TX, NT = Y::NT;
@@ -867,12 +876,14 @@ struct ZArgs {
     int mean_kx, mean_jp;             // real (0,0) column
     double mean_scale;                // 1/(fx fy n1m n2m n3m)
     int zper;                         // periodic z: ghost planes wrap instead of wall extrapolation
+    const int *gate;                  // refinement gate (gated_off), nullptr = always run
 };
 
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_zthomas(T *__restrict__ W, T *__restrict__ cs, ZArgs z, const double *__restrict__ dx3,
           const double *__restrict__ dmx3, double *__restrict__ mean_out) {
+    if (gated_off(z.gate)) return;
     extern __shared__ double zsh[];
     double *am = zsh, *ap = zsh + z.n3m;
     for (int k = threadIdx.x; k < z.n3m; k += blockDim.x) {
@@ -987,17 +998,6 @@ __device__ __forceinline__ double frcp(double x) {
     return r;
 }
 
-// z row coefficients for one solve as a table (read by the residual forms and
-// the pipelined z kernel): zam[k], zap[k] (k < n3m), then zap[n3m..n3m+3] =
-// the wall extrapolation coefficients.
-__global__ void __launch_bounds__(256)
-k_zcoef(const double *__restrict__ dx3, const double *__restrict__ dmx3, int n3m, double *__restrict__ zam,
-        double *__restrict__ zap) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n3m) z_row(dx3, dmx3, n3m, k, zam[k], zap[k]);
-    if (k == 0) z_wall_bc(dmx3, n3m, zap + n3m);
-}
-
 // WPE: minimum waves per SIMD the register allocation must allow (occupancy
 // is what lets one workgroup's loads overlap another's arithmetic here).
 // CT: arithmetic type (FP64 for every plan: FP32 arithmetic on FP32 storage
@@ -1006,6 +1006,7 @@ template <typename T, int K, int S, int NTMAX, int WPE, typename CT = double>
 __global__ void __launch_bounds__(NTMAX) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double *__restrict__ zap,
         double *__restrict__ mean_out, int P) {
+    if (gated_off(z.gate)) return;
     extern __shared__ double zsh[];
     const int n3m = z.n3m;
     const int NP = P * K;           // padded rows
@@ -1260,6 +1261,7 @@ template <typename T, int K, int S, int WPE>
 __global__ void __launch_bounds__(S * 32) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double *__restrict__ zap,
         double *__restrict__ mean_out, int P) {
+    if (gated_off(z.gate)) return;
     constexpr int RS = 65;  // LDS column stride of the reduced system (2P <= 64, padded)
     extern __shared__ double zsh[];
     const int n3m = z.n3m;
@@ -1272,7 +1274,12 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     double *red_d = red_u + S * RS;
     double *ps = red_d + S * RS;   // [P][S] partial sums
     const int tid = threadIdx.x, s = tid % S, p = tid / S;
-    const int lane = tid & 63, wv = tid >> 6, nwv = (int)((blockDim.x + 63) >> 6);
+    // A column's reduced system goes to one wavefront, lane i holding row i,
+    // so only waves with at least R lanes take columns: every full wave, and
+    // the last (partial) wave only if it still has R lanes.  R = 2P <= S P, so
+    // wave 0 always qualifies.
+    const int lane = tid & 63, wv = tid >> 6;
+    const int nwv = (int)(blockDim.x >> 6) + ((int)(blockDim.x & 63) >= 2 * P ? 1 : 0);
     for (int k = tid; k < NP; k += blockDim.x) {
         am[k] = k < n3m ? zam[k] : 0.0;
         ap[k] = k < n3m ? zap[k] : 0.0;
@@ -1367,7 +1374,8 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
         red_d[s * RS + 2 * p + 1] = Df;
         __syncthreads();
         // cyclic reduction of each column's reduced system inside one wavefront
-        for (int col = wv; col < S; col += nwv) {
+        // (waves past nwv take none; a shuffle source past R is masked below)
+        for (int col = wv; wv < nwv && col < S; col += nwv) {
             const bool in = lane < R;
             double l_ = in ? red_l[col * RS + lane] : 0.0;
             double u_ = in ? red_u[col * RS + lane] : 0.0;
@@ -1462,7 +1470,8 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
 // ---------------------------------------------------------------------------
 template <int KZ>
 __global__ void __launch_bounds__(256)
-k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m) {
+k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m, const int *__restrict__ gate) {
+    if (gated_off(gate)) return;
     __shared__ double red[256];
     const int nxc = (n1m + 255) / 256, nkb = (n3m + KZ - 1) / KZ;
     const long ntiles = (long)nxc * n2m * nkb;
@@ -1527,66 +1536,6 @@ k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m) {
         xr.partial[3 * blockIdx.x + 1] = tf;
         xr.partial[3 * blockIdx.x + 2] = tx;
     }
-}
-
-// Totals of the per-workgroup triples (sum r^2, sum f^2, sum x) in a fixed
-// order (deterministic).
-__global__ void __launch_bounds__(256) k_reduce3(const double *__restrict__ partial, int n, double *__restrict__ out) {
-    __shared__ double red[256];
-    double a = 0.0, b = 0.0, c = 0.0;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        a += partial[3 * q];
-        b += partial[3 * q + 1];
-        c += partial[3 * q + 2];
-    }
-    a = block_sum<256>(a, red);
-    b = block_sum<256>(b, red);
-    c = block_sum<256>(c, red);
-    if (threadIdx.x == 0) {
-        out[0] = a;
-        out[1] = b;
-        out[2] = c;
-    }
-}
-
-// Ghost layers of P from its interior, one face pair per launch:
-// wall extrapolation p0 = BC_a p1 - BC_b p2 (cuda_neumann_BC /
-// apply_bc_on_direction, fft_3d_poisson.f90:798-852) or periodic wrap
-// (cuda_ghostcell_update on one rank, :666-796), over the full (ghosted)
-// ranges of the other two axes.  The reference's order -- walls along x, y,
-// z, then wraps along x, y, z -- is the caller's launch order.
-__global__ void __launch_bounds__(256)
-k_ghost_face(double *__restrict__ P, int n1m, int n2m, int n3m, int dir, int wrap, const double *__restrict__ dmx) {
-    const size_t L1 = (size_t)n1m + 2, L2 = (size_t)n2m + 2, L3 = (size_t)n3m + 2;
-    const size_t nb = dir == 0 ? L2 : L1, nc = dir == 2 ? L2 : L3;
-    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nb * nc) return;
-    const size_t b = q % nb, c = q / nb;
-    const int nm = dir == 0 ? n1m : (dir == 1 ? n2m : n3m);
-    // element (b, c, position along dir)
-    size_t base, step;
-    if (dir == 0) { base = (c * L2 + b) * L1; step = 1; }
-    else if (dir == 1) { base = c * L1 * L2 + b; step = L1; }
-    else { base = c * L1 + b; step = L1 * L2; }
-    double *x = P + base;
-    if (wrap) {
-        const double lo = x[1 * step], hi = x[(size_t)nm * step];
-        x[0] = hi;
-        x[(size_t)(nm + 1) * step] = lo;
-    } else {
-        const double d1 = dmx[1], d2 = dmx[2], s0 = d1 + d2;
-        const double e1 = dmx[nm + 1], e2 = dmx[nm], s1 = e1 + e2;
-        x[0] = (s0 * s0 / (s0 * s0 - d1 * d1)) * x[step] - (d1 * d1 / (s0 * s0 - d1 * d1)) * x[2 * step];
-        x[(size_t)(nm + 1) * step] = (s1 * s1 / (s1 * s1 - e1 * e1)) * x[(size_t)nm * step] -
-                                     (e1 * e1 / (s1 * s1 - e1 * e1)) * x[(size_t)(nm - 1) * step];
-    }
-}
-
-// Plain streaming copy (16-B lanes, grid-stride): the measured-bandwidth
-// yardstick for the pass kernels and the calibration of HBM byte counters.
-__global__ void __launch_bounds__(256) k_copy(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
 }  // namespace ppf

@@ -7,7 +7,7 @@ against the oracle. Eager host work (zeroing P, synchronisation) runs between
 replays.
 
 Tolerances: FP64 problems match the oracle to <= 1e-12 relative L2. FP32_IR
-problems reach the relative residual 1e-10 and are <= 1e-8 from the oracle.
+problems reach the relative residual 1e-10 and are <= 1e-10 from the oracle.
 A graph replay reproduces the eager result bitwise: it runs the same kernels on
 the same data.
 """
@@ -62,7 +62,7 @@ def test_batch_eager_then_graph_replays(flags):
             if kw["precision"] == api.PPF_PREC_FP64:
                 assert sw == 0 and _rel(P, ref) <= 1e-12
             else:
-                assert 1 <= sw <= 5 and res <= 1e-10 and _rel(P, ref) <= 1e-8
+                assert 1 <= sw <= 5 and res <= 1e-10 and _rel(P, ref) <= 1e-10
         for rep in range(3):
             for _, dev, _ in probs:
                 dev.p.fill_zero()
@@ -98,3 +98,32 @@ def test_tune_picks_a_variant_and_keeps_parity():
     finally:
         dev.free()
         plan.clean()
+
+
+def test_fp32_ir_replay_on_a_rhs_needing_more_sweeps():
+    """The graph is captured after an eager solve of a zero right-hand side,
+    which needs no sweep.  A replay on a real right-hand side must still reach
+    the tolerance (the device-side test adds the sweeps) -- or report
+    PPF_ERR_NOT_CONVERGED -- never return silently above it."""
+    n1, n2, n3 = 48, 32, 40
+    L = (1.0, 1.0, 1.0)
+    ax, ay = make_axis(n1, L[0], False), make_axis(n2, L[1], False)
+    az = make_axis(n3, L[2], False, uniform=False, gamma=1.5)
+    f = W.remove_weighted_mean(np.random.default_rng(5).standard_normal((n3, n2, n1)), az)
+    dev = api.Problem3D(np.zeros_like(f), ax, ay, az)
+    kw = dict(n1m=n1, n2m=n2, n3m=n3, L=L, pbc=(False, False, False), precision=api.PPF_PREC_FP32_IR,
+              tol=1e-10, max_sweeps=8)
+    b = api.Batch([(kw, dev)])
+    try:
+        (sw0, _), = b.solve(stats=True)
+        assert sw0 == 0
+        dev.rhs.upload(f)
+        b.solve()
+        (sw, res), = b.solve(stats=True)
+        P = dev.p.download()
+        ref = O.solve(f, L, (False, False, False), ax.dmx, ay.dmx, az.dmx, az.dx)
+        print(f"learned sweeps={sw0}; replay sweeps={sw} res={res:.3e} err={_rel(P, ref):.3e}")
+        assert sw >= 1 and res <= 1e-10 and _rel(P, ref) <= 1e-10
+    finally:
+        b.destroy()
+        dev.free()

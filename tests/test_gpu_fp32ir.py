@@ -5,9 +5,8 @@ correction x += e, the pinned (0,0) z system and the mean stay FP64.  Pass
 criteria:
   * the solve stops with ||f - L x||/||f|| <= tol = 1e-10. The library's own
     FP64 residual kernel recomputes it independently (<= 1.05e-10);
-  * relative L2 of the full ghosted P against the FP64 oracle <= 1e-8.  That is
-    the residual target times the operator's error amplification on these
-    grids; the measured value is printed;
+  * relative L2 of the full ghosted P against the FP64 oracle <= 1e-10 (the
+    measured values are about 1e-12 to 1e-11 and are printed);
   * the sweep count stays small (<= 5), as contraction by an FP32 solve implies.
 """
 import numpy as np
@@ -21,7 +20,7 @@ from pascal_poisson_fft_amd.grid import make_axis
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
-ERR_VS_ORACLE = 1e-8
+ERR_VS_ORACLE = 1e-10
 
 
 def _rel_l2(a, b):
@@ -111,3 +110,52 @@ def test_fp32_ir_config_shape_256():
     print(f"256^3 fp32_ir sweeps={st.sweeps} rel_res={st.rel_residual:.3e} err_vs_oracle={err:.3e}")
     assert st.rel_residual <= TOL and res <= 1.05 * TOL
     assert err <= ERR_VS_ORACLE
+
+
+def _plan_dev(ax, ay, az, f, **kw):
+    n3, n2, n1 = f.shape
+    plan = api.FFTPoisson(n1, n2, n3, (ax.L, ay.L, az.L), (ax.pbc, ay.pbc, False),
+                          precision=api.PPF_PREC_FP32_IR, **kw)
+    return plan, api.Problem3D(f, ax, ay, az)
+
+
+@pytest.mark.parametrize("pbc", [(False, False), (True, False)], ids=["NN", "PN"])
+def test_fp32_ir_device_side_convergence(pbc):
+    """ppf_solve_async with min_sweeps = 0: the device-side residual test alone
+    decides the sweeps; the result matches the host-tested solve."""
+    ax, ay, az, f = _setup(64, 48, 40, pbc, False, 1.5, seed=12)
+    P_host, st_host, _ = _run(ax, ay, az, f, tol=TOL, max_sweeps=8)
+    plan, dev = _plan_dev(ax, ay, az, f, tol=TOL, max_sweeps=8)
+    try:
+        plan.solve_async(dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3, min_sweeps=0)
+        st = plan.async_result()
+        res = plan.residual(dev.rhs, dev.p, dev.dmx3, dev.dx3)
+        P = dev.p.download()
+        print(f"async sweeps={st.sweeps} res={st.rel_residual:.3e} recomputed={res:.3e}; host-tested "
+              f"sweeps={st_host.sweeps}")
+        assert st.sweeps == st_host.sweeps >= 1
+        assert st.rel_residual <= TOL and res <= 1.05 * TOL
+        assert np.array_equal(P, P_host)
+        # more unconditional sweeps than needed: the device test runs none extra
+        plan.solve_async(dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3, min_sweeps=st.sweeps + 1)
+        st2 = plan.async_result()
+        assert st2.sweeps == st.sweeps + 1 and st2.rel_residual <= TOL
+    finally:
+        dev.free()
+        plan.clean()
+
+
+def test_fp32_ir_device_side_not_converged():
+    """A tolerance below FP64 reach: every sweep runs and the result reports the miss."""
+    ax, ay, az, f = _setup(32, 32, 16, (True, False), False, 1.2, seed=4)
+    plan, dev = _plan_dev(ax, ay, az, f, tol=1e-30, max_sweeps=3)
+    try:
+        plan.solve_async(dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3, min_sweeps=0)
+        with pytest.raises(api.PPFError) as ei:
+            plan.async_result()
+        assert ei.value.code == api.PPF_ERR_NOT_CONVERGED
+        st = plan.async_result(allow_not_converged=True)
+        assert st.sweeps == 3 and 0 < st.rel_residual < 1e-9
+    finally:
+        dev.free()
+        plan.clean()

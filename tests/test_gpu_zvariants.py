@@ -1,0 +1,82 @@
+"""Every z-pass kernel shape, pinned with ppf_plan_set_zvariant, against the
+FP64 oracle.
+
+The autotuner may return any of these shapes, so each one must be correct on
+its own, at z extents that stress its partitioning: 96 (the tuner test's
+extent), 136 and 144 (partial last wavefront for the pipelined K=16 forms),
+270 (partial wavefront for K=32 pipelined forms, ragged last partition), 768
+(the projection stream's extent), 1500 (above 1024, where the default is the
+K=32, S=8 shape) -- and 2050, where only the one-thread-per-line Thomas
+runs.  Both z-row layouts are covered: Neumann y (rows 1..n2m) and periodic y
+with Neumann x (R2C columns, rows 0..n2m+1).
+
+A shape that does not fit an extent (its partitions exceed the workgroup or
+its reduced system the wavefront) is rejected with PPF_ERR_UNSUPPORTED, never
+run; the test records which.  Tolerance: <= 1e-12 relative L2 of the ghosted
+solution against the oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pascal_poisson_fft_amd import api
+from pascal_poisson_fft_amd.grid import make_axis
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = list(range(1, 15))
+UNSUPPORTED = -5
+
+
+def _case(n1, n2, n3, pbc, seed):
+    L = (2.0, 3.0, 1.5)
+    ax, ay = make_axis(n1, L[0], pbc[0]), make_axis(n2, L[1], pbc[1])
+    az = make_axis(n3, L[2], False, uniform=False, gamma=1.6)
+    f = np.random.default_rng(seed).standard_normal((n3, n2, n1))
+    w = az.dx[1:n3 + 1]
+    f -= float(np.einsum("kji,k->", f, w)) / (w.sum() * n1 * n2)
+    ref = O.solve(f, L, (pbc[0], pbc[1], False), ax.dmx, ay.dmx, az.dmx, az.dx)
+    return L, ax, ay, az, f, ref
+
+
+@pytest.mark.parametrize("pbc", [(False, False), (False, True)], ids=["NN", "NP"])
+@pytest.mark.parametrize("n3", [96, 136, 144, 270, 768, 1500])
+def test_every_zvariant_matches_oracle(pbc, n3):
+    n1, n2 = 24, 16
+    L, ax, ay, az, f, ref = _case(n1, n2, n3, pbc, seed=n3 + 7 * pbc[1])
+    plan = api.FFTPoisson(n1, n2, n3, L, (pbc[0], pbc[1], False))
+    dev = api.Problem3D(f, ax, ay, az)
+    ran, rejected = [], []
+    try:
+        for v in VARIANTS:
+            print(f"n3m={n3} pbc={pbc} zvariant={v}", flush=True)
+            plan.set_zvariant(v)
+            dev.p.fill_zero()
+            try:
+                plan.solve(dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3)
+            except api.PPFError as e:
+                assert e.code == UNSUPPORTED, e
+                rejected.append(v)
+                continue
+            api.sync()
+            got = dev.p.download()
+            err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+            assert err <= 1e-12, (v, err)
+            ran.append(v)
+    finally:
+        dev.free()
+        plan.clean()
+    print(f"  ran {ran}; unsupported at this extent {rejected}")
+    # the default shapes and the one-thread-per-line kernel fit every extent here
+    assert 3 in ran and 6 in ran
+
+
+@pytest.mark.parametrize("pbc", [(False, False), (True, False)], ids=["NN", "PN"])
+def test_long_z_lines_use_one_thread_per_line(pbc):
+    """n3m = 2050 > 2048: the default is the one-thread-per-line Thomas."""
+    n1, n2, n3 = 16, 8, 2050
+    print(f"n3m={n3} pbc={pbc} default z kernel", flush=True)
+    L, ax, ay, az, f, ref = _case(n1, n2, n3, pbc, seed=3)
+    got = api.solve_host(f, ax, ay, az)
+    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    assert err <= 1e-12, err
