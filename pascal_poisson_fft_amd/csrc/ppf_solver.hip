@@ -21,9 +21,7 @@
 
 using namespace ppf;
 
-namespace {
-const double kPi = 3.14159265358979323846264338327950288;
-}  // namespace
+constexpr double kPi = 3.14159265358979323846264338327950288;
 
 template <typename T>
 static int dev_alloc(ppf_plan *p, T **ptr, size_t count) {
@@ -64,13 +62,79 @@ static int upload_tables(ppf_plan *p, AxisTables<T> &tb, int N, int kind) {
     return PPF_OK;
 }
 
-// modified wavenumbers, fft_3d_poisson.f90:288-351
-static double eig_neumann(int m, int n, double h) {
-    double s = std::sin(0.5 * (double)m * kPi / (double)n);
+// modified wavenumbers, fft_3d_poisson.f90:288-351 (host for plans, device
+// for the allocation-free solve: one formula)
+__host__ __device__ static inline double eig_neumann(int m, int n, double h) {
+    const double s = sin(0.5 * (double)m * kPi / (double)n);
     return 4.0 * s * s / (h * h);
 }
-static double eig_periodic(int m, int n, double h) {
-    return 2.0 * (1.0 - std::cos(2.0 * (double)m * kPi / (double)n)) / (h * h);
+__host__ __device__ static inline double eig_periodic(int m, int n, double h) {
+    return 2.0 * (1.0 - cos(2.0 * (double)m * kPi / (double)n)) / (h * h);
+}
+
+// Eigenvalue and zero-mode flag of workspace scalar column kx (x) and row jp
+// (y).  Periodic x stores the half spectrum as re/im pairs (column kx is mode
+// kx/2); y rows follow the y pass: DCT rows 1..n2m are modes 0..n2m-1, C2C
+// rows 1..n2m are the wrapped modes of the complex FFT, R2C rows 0..n2m+1 are
+// re/im pairs of modes 0..n2m/2.  Unused ghost rows get 0.
+struct EigGeom {
+    int px, ymode, n1m, n2m;
+    double h1, h2;
+};
+__host__ __device__ static inline void eig_col(const EigGeom &g, int kx, double &lam, unsigned char &zero) {
+    const int m = g.px ? kx / 2 : kx;
+    lam = g.px ? eig_periodic(m, g.n1m, g.h1) : eig_neumann(m, g.n1m, g.h1);
+    zero = (m == 0);
+}
+__host__ __device__ static inline void eig_row(const EigGeom &g, int jp, double &lam, unsigned char &zero) {
+    int m;
+    lam = 0.0;
+    zero = 0;
+    if (g.ymode == 0) m = jp - 1;
+    else if (g.ymode == 1) { const int j = jp - 1; m = (j <= g.n2m / 2) ? j : g.n2m - j; }
+    else m = jp / 2;
+    if (m < 0 || m > g.n2m) return;
+    if (g.ymode == 0 && jp > g.n2m) return;
+    if (g.ymode == 1 && (jp < 1 || jp > g.n2m)) return;
+    lam = (g.ymode == 0) ? eig_neumann(m, g.n2m, g.h2) : eig_periodic(m, g.n2m, g.h2);
+    zero = (m == 0);
+}
+
+// Checks of a problem description shared by plans and the allocation-free solve.
+static int desc_check(const ppf_desc &d) {
+    if (d.n1m <= 0 || d.n2m <= 0 || d.n3m <= 0 || !(d.L1 > 0) || !(d.L2 > 0) || !(d.L3 > 0)) return PPF_ERR_INVALID_ARG;
+    if (d.precision != PPF_PREC_FP64 && d.precision != PPF_PREC_FP32_IR) return PPF_ERR_INVALID_ARG;
+    // every n must be even (01_Fortran/examples/global.f90:114-120)
+    // z rows live in LDS in the z pass (2 doubles per row): n3m <= 4096
+    if (!size_supported(d.n1m) || !size_supported(d.n2m) || d.n3m < 2 || (d.n3m & 1) || d.n3m > 4096)
+        return PPF_ERR_SIZE;
+    return PPF_OK;
+}
+
+// Workspace geometry, transform kinds and mean bookkeeping of a problem.
+static void plan_geometry(const ppf_desc &d, ppf_plan *p) {
+    p->d = d;
+    const bool px = d.pbc1 != 0, py = d.pbc2 != 0;
+    p->xkind = px ? AX_R2C : AX_DCT;
+    if (!py) { p->ykind = AX_DCT; p->ymode = 0; }
+    else if (px) { p->ykind = AX_C2C; p->ymode = 1; }
+    else { p->ykind = AX_R2C; p->ymode = 2; }
+    p->Lx = px ? d.n1m + 2 : d.n1m;
+    p->Ly = d.n2m + 2;
+    p->Lz = d.n3m + 2;
+    if (p->ymode == 2) { p->zjlo = 0; p->znjr = d.n2m + 2; }
+    else { p->zjlo = 1; p->znjr = d.n2m; }
+    p->mean_kx = 0;
+    p->mean_jp = (p->ymode == 2) ? 0 : 1;
+    p->zvariant = 0;
+    const double fx = px ? 1.0 : 2.0, fy = (p->ymode == 0) ? 2.0 : 1.0;
+    p->mean_scale = 1.0 / (fx * fy * (double)d.n1m * (double)d.n2m * (double)d.n3m);
+    const bool fp32 = d.precision == PPF_PREC_FP32_IR;
+    p->Lxs = w_stride(p->Lx, fp32 ? (int)sizeof(float) : (int)sizeof(double));
+}
+
+static EigGeom eig_geom(const ppf_plan *p) {
+    return EigGeom{p->d.pbc1 != 0, p->ymode, p->d.n1m, p->d.n2m, p->d.L1 / p->d.n1m, p->d.L2 / p->d.n2m};
 }
 
 static void free_plan(ppf_plan *p) {
@@ -107,35 +171,12 @@ int ppf_plan_create(const ppf_desc *desc, ppf_plan **out) {
     if (!desc || !out) return PPF_ERR_INVALID_ARG;
     *out = nullptr;
     const ppf_desc &d = *desc;
-    if (d.n1m <= 0 || d.n2m <= 0 || d.n3m <= 0 || !(d.L1 > 0) || !(d.L2 > 0) || !(d.L3 > 0)) return PPF_ERR_INVALID_ARG;
-    if (d.precision != PPF_PREC_FP64 && d.precision != PPF_PREC_FP32_IR) return PPF_ERR_INVALID_ARG;
-    // every n must be even (01_Fortran/examples/global.f90:114-120)
-    // z rows live in LDS in the z pass (2 doubles per row): n3m <= 4096
-    if (!size_supported(d.n1m) || !size_supported(d.n2m) || d.n3m < 2 || (d.n3m & 1) || d.n3m > 4096)
-        return PPF_ERR_SIZE;
-
+    int rc = desc_check(d);
+    if (rc) return rc;
     ppf_plan *p = new (std::nothrow) ppf_plan();
     if (!p) return PPF_ERR_ALLOC;
-    p->d = d;
-    const bool px = d.pbc1 != 0, py = d.pbc2 != 0;
-    p->xkind = px ? AX_R2C : AX_DCT;
-    if (!py) { p->ykind = AX_DCT; p->ymode = 0; }
-    else if (px) { p->ykind = AX_C2C; p->ymode = 1; }
-    else { p->ykind = AX_R2C; p->ymode = 2; }
-    p->Lx = px ? d.n1m + 2 : d.n1m;
-    p->Ly = d.n2m + 2;
-    p->Lz = d.n3m + 2;
-    if (p->ymode == 2) { p->zjlo = 0; p->znjr = d.n2m + 2; }
-    else { p->zjlo = 1; p->znjr = d.n2m; }
-    p->mean_kx = 0;
-    p->mean_jp = (p->ymode == 2) ? 0 : 1;
-    p->zvariant = 0;
-    const double fx = px ? 1.0 : 2.0, fy = (p->ymode == 0) ? 2.0 : 1.0;
-    p->mean_scale = 1.0 / (fx * fy * (double)d.n1m * (double)d.n2m * (double)d.n3m);
-
-    int rc = PPF_OK;
+    plan_geometry(d, p);
     const bool fp32 = d.precision == PPF_PREC_FP32_IR;
-    p->Lxs = w_stride(p->Lx, fp32 ? (int)sizeof(float) : (int)sizeof(double));
     const size_t wsz = (size_t)p->Lxs * p->Ly * p->Lz;
     // Thomas scratch only where the one-thread-per-line kernel runs by
     // default (n3m > 2048); ppf_plan_set_zvariant allocates it when that
@@ -165,25 +206,11 @@ int ppf_plan_create(const ppf_desc *desc, ppf_plan **out) {
     }
 
     // eigenvalue and zero-mode tables per scalar column / row
-    const double h1 = d.L1 / d.n1m, h2 = d.L2 / d.n2m;
+    const EigGeom eg = eig_geom(p);
     std::vector<double> lx(p->Lx), ly(p->Ly, 0.0);
     std::vector<unsigned char> zx(p->Lx, 0), zy(p->Ly, 0);
-    for (int kx = 0; kx < p->Lx; ++kx) {
-        const int m = px ? kx / 2 : kx;
-        lx[kx] = px ? eig_periodic(m, d.n1m, h1) : eig_neumann(m, d.n1m, h1);
-        zx[kx] = (m == 0);
-    }
-    for (int jp = 0; jp < p->Ly; ++jp) {
-        int m;
-        if (p->ymode == 0) m = jp - 1;
-        else if (p->ymode == 1) { const int j = jp - 1; m = (j <= d.n2m / 2) ? j : d.n2m - j; }
-        else m = jp / 2;
-        if (m < 0 || m > d.n2m) { ly[jp] = 0.0; zy[jp] = 0; continue; }  // unused ghost rows
-        if (p->ymode == 0 && jp > d.n2m) { ly[jp] = 0.0; zy[jp] = 0; continue; }
-        if (p->ymode == 1 && (jp < 1 || jp > d.n2m)) { ly[jp] = 0.0; zy[jp] = 0; continue; }
-        ly[jp] = (p->ymode == 0) ? eig_neumann(m, d.n2m, h2) : eig_periodic(m, d.n2m, h2);
-        zy[jp] = (m == 0);
-    }
+    for (int kx = 0; kx < p->Lx; ++kx) eig_col(eg, kx, lx[kx], zx[kx]);
+    for (int jp = 0; jp < p->Ly; ++jp) eig_row(eg, jp, ly[jp], zy[jp]);
     if (hipMemcpy(p->lamx, lx.data(), sizeof(double) * p->Lx, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->lamy, ly.data(), sizeof(double) * p->Ly, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->zx, zx.data(), p->Lx, hipMemcpyHostToDevice) != hipSuccess ||

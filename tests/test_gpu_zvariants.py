@@ -12,8 +12,15 @@ with Neumann x (R2C columns, rows 0..n2m+1).
 
 A shape that does not fit an extent (its partitions exceed the workgroup or
 its reduced system the wavefront) is rejected with PPF_ERR_UNSUPPORTED, never
-run; the test records which.  Tolerance: <= 1e-12 relative L2 of the ghosted
-solution against the oracle.
+run; the test records which.
+
+Tolerance: <= 1e-12 relative L2 of the ghosted solution against the oracle
+for z lines of up to 1024 rows.  Longer stretched lines are worse conditioned
+than that bar: the oracle's own FP64 Thomas on the pinned (0,0) mode of a
+1500-row tanh-1.6 line differs from an 80-bit extended-precision solve by
+5.5e-12 (7.0e-12 at 2050 rows), so two correct FP64 algorithms may differ by
+that much; there the bar is 1e-11.  Every solve must also leave an FP64
+relative residual <= 1e-12, which no conditioning excuses.
 """
 import numpy as np
 import pytest
@@ -26,6 +33,10 @@ pytestmark = pytest.mark.gpu
 
 VARIANTS = list(range(1, 15))
 UNSUPPORTED = -5
+
+
+def _tol(n3):
+    return 1e-12 if n3 <= 1024 else 1e-11
 
 
 def _case(n1, n2, n3, pbc, seed):
@@ -61,7 +72,8 @@ def test_every_zvariant_matches_oracle(pbc, n3):
             api.sync()
             got = dev.p.download()
             err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-            assert err <= 1e-12, (v, err)
+            res = plan.residual(dev.rhs, dev.p, dev.dmx3, dev.dx3)
+            assert err <= _tol(n3) and res <= 1e-12, (v, err, res)
             ran.append(v)
     finally:
         dev.free()
@@ -79,4 +91,5 @@ def test_long_z_lines_use_one_thread_per_line(pbc):
     L, ax, ay, az, f, ref = _case(n1, n2, n3, pbc, seed=3)
     got = api.solve_host(f, ax, ay, az)
     err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-    assert err <= 1e-12, err
+    res = O.rel_residual(f, got, L, (pbc[0], pbc[1]), az.dmx, az.dx)
+    assert err <= _tol(n3) and res <= 1e-12, (err, res)

@@ -37,6 +37,31 @@ def test_grid_tanh_invariants(nm, gamma):
     assert abs(dmx[1] - 0.5 * dx[1]) < 1e-16 and abs(dmx[nm + 1] - 0.5 * dx[nm]) < 1e-16
 
 
+# Nodes x_0..x_9 of the tanh channel grid for nm = 8, L = 2, gamma = 1.5,
+# x_i = L/2 (1 + tanh(gamma/2 (2(i-1)/nm - 1)) / tanh(gamma/2))
+# (mpi_subdomain.f90:424-425), evaluated by hand at 40 digits (tanh via exp in
+# decimal arithmetic) and rounded to 20; x_0 is the formula's value below the
+# wall, which the wall then replaces by x_1 (:413).
+TANH_NODES_NM8_G15 = [-0.15574703673089792187, 0.0, 0.19730643997916327275, 0.43578998752362805896,
+                      0.70820513957939889936, 1.0, 1.29179486042060110064, 1.56421001247637194104,
+                      1.80269356002083672725, 2.0]
+
+
+@pytest.mark.parametrize("pbc", [False, True])
+def test_grid_tanh_literal_nodes(pbc):
+    """A literal fixture: catches a slip in the stretch law (gamma for gamma/2,
+    a lost factor 2, a shifted index) that the symmetry and sum invariants
+    above cannot see."""
+    want = np.array(TANH_NODES_NM8_G15)
+    if not pbc:
+        want[0] = want[1]
+    x, dx, dmx = O.grid(False, 8, 2.0, 1.5, pbc)
+    np.testing.assert_allclose(x, want, rtol=0, atol=4e-16)
+    a = make_axis(8, 2.0, pbc, uniform=False, gamma=1.5)
+    np.testing.assert_allclose(a.x, want, rtol=0, atol=4e-16)
+    np.testing.assert_allclose(dx[1:9], np.diff(want[1:]), rtol=0, atol=1e-15)
+
+
 def test_grid_uniform_and_periodic():
     x, dx, dmx = O.grid(True, 16, 2 * np.pi, 0.0, True)
     h = 2 * np.pi / 16
