@@ -32,6 +32,7 @@ extern "C" {
 #define PPF_ERR_ALLOC (-4)         /* device allocation failed */
 #define PPF_ERR_UNSUPPORTED (-5)   /* a configuration not implemented */
 #define PPF_ERR_NOT_CONVERGED (-6) /* refinement hit max_sweeps above tol */
+#define PPF_ERR_SCRATCH (-7)       /* beyond the static scratch of ppf_solve_noalloc */
 
 #define PPF_PREC_FP64 0    /* FP64 transforms and Thomas */
 #define PPF_PREC_FP32_IR 1 /* FP32 transforms/Thomas + FP64 residual refinement */
@@ -94,6 +95,21 @@ int ppf_solve(ppf_plan *plan, const double *prhs, double *p, const double *dmx1,
  * -B_y, -B_x (fft_3d_poisson.f90:377-624), for measurement. */
 int ppf_solve_timed(ppf_plan *plan, const double *prhs, double *p, const double *dmx1, const double *dmx2,
                     const double *dmx3, const double *dx3, void *stream, float stage_ms[5]);
+
+/* ppf_solve without a plan and without allocating: for callers that cannot
+ * call hipMalloc (code under hipGraph capture, tight serving loops).  No
+ * counterpart in the reference.  FP64 descriptors only (PPF_ERR_UNSUPPORTED
+ * otherwise); same arguments, result and ghost layers as ppf_solve with a plan
+ * of *desc.  The workspace, tables and scalars live in a static device array
+ * of 64 MiB that each GPU holds once for the life of the process; a problem
+ * whose workspace ((n1m or n1m+2 rounded up to 16) x (n2m+2) x (n3m+2)
+ * doubles) and tables exceed it returns PPF_ERR_SCRATCH (e.g. 128 x 128 x 256
+ * fits, 256 x 256 x 128 does not).  Only kernel launches on `stream`: the axis
+ * tables are rebuilt on the device each call.  Calls on one GPU share the
+ * array, so calls that overlap on one GPU (other streams, other threads)
+ * race; serialise them. */
+int ppf_solve_noalloc(const ppf_desc *desc, const double *prhs, double *p, const double *dmx1, const double *dmx2,
+                      const double *dmx3, const double *dx3, void *stream);
 
 /* Frees everything the plan owns.  Replaces cuda_Poisson_FFT_clean
  * (fft_3d_poisson.f90:123-130). */

@@ -27,6 +27,7 @@ LIB_PATH = os.path.join(_PKG, "lib", "libpascal_poisson_fft.so")
 PPF_PREC_FP64 = 0
 PPF_PREC_FP32_IR = 1
 PPF_ERR_NOT_CONVERGED = -6
+PPF_ERR_SCRATCH = -7
 
 H2D, D2H, D2D = 1, 2, 3
 
@@ -94,6 +95,7 @@ def lib() -> C.CDLL:
     L.ppf_copy_timed.argtypes = [vp, vp, sz, I, vp, C.POINTER(C.c_float)]
     L.ppf_solve_sweeps.argtypes = [vp, vp, vp, vp, vp, vp, vp, I, vp]
     L.ppf_solve_async.argtypes = [vp, vp, vp, vp, vp, vp, vp, I, vp]
+    L.ppf_solve_noalloc.argtypes = [C.POINTER(Desc), vp, vp, vp, vp, vp, vp, vp]
     L.ppf_solve_async_result.argtypes = [vp, vp, C.POINTER(Stats)]
     L.ppf_plan_tune.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(I)]
     L.ppf_plan_set_zvariant.argtypes = [vp, I]
@@ -147,6 +149,18 @@ def copy_bandwidth(count: int = 1 << 28, reps: int = 10) -> float:
     finally:
         a.free()
         b.free()
+
+
+def solve_noalloc(n1m: int, n2m: int, n3m: int, L, pbc, prhs: "DeviceArray", p: "DeviceArray",
+                  dmx1: "DeviceArray", dmx2: "DeviceArray", dmx3: "DeviceArray", dx3: "DeviceArray",
+                  stream=None) -> None:
+    """FP64 solve without a plan and without device allocation (ppf_solve_noalloc):
+    the library's static device scratch holds the workspace; calls on one GPU
+    must not overlap."""
+    d = Desc(int(n1m), int(n2m), int(n3m), float(L[0]), float(L[1]), float(L[2]),
+             int(bool(pbc[0])), int(bool(pbc[1])), int(bool(pbc[2])), PPF_PREC_FP64, 0.0, 0)
+    _check(lib().ppf_solve_noalloc(C.byref(d), prhs.ptr, p.ptr, dmx1.ptr, dmx2.ptr, dmx3.ptr, dx3.ptr, stream),
+           "ppf_solve_noalloc")
 
 
 class DeviceArray:

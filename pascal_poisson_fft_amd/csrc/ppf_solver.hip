@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -163,6 +164,9 @@ const char *ppf_error_string(int code) {
         case PPF_ERR_ALLOC: return "device allocation failed";
         case PPF_ERR_UNSUPPORTED: return "unsupported configuration";
         case PPF_ERR_NOT_CONVERGED: return "refinement did not reach the tolerance";
+        case PPF_ERR_SCRATCH:
+            return "problem too large for the static device scratch of ppf_solve_noalloc (64 MiB: workspace "
+                   "(n1m or n1m+2)(n2m+2)(n3m+2) doubles, rows padded to 128 bytes, plus tables)";
     }
     return "unknown error";
 }
@@ -565,6 +569,156 @@ int ppf_residual(ppf_plan *p, const double *f, const double *P, const double *dm
     *rel = p->hred[1] > 0 ? std::sqrt(p->hred[0] / p->hred[1]) : std::sqrt(p->hred[0]);
     return PPF_OK;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free solve (ppf_solve_noalloc), for callers that cannot allocate
+// device memory: the workspace, axis tables, z row table and scalars of one
+// FP64 solve live in `scratch`, a file-scope device array every GPU holds once
+// for the life of the process.  Calls on one GPU share it (and keep the last
+// call's contents), so calls that overlap on one GPU race.  Nothing in it has
+// to start at zero: every pass writes what the next one reads.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};  // device address of `scratch`, per GPU
+
+// The device address of the current GPU's scratch array, looked up once per
+// GPU (the array's name is a host-side stand-in, not a device address).
+int scratch_base(double **out) {
+    int dev = 0;
+    PPF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return PPF_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return PPF_ERR_HIP;
+        g_scratch[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch[dev];
+    return PPF_OK;
+}
+
+// 256-byte aligned slices of the scratch array
+struct ScratchCarver {
+    char *base;
+    size_t used = 0;
+    template <typename T>
+    T *take(size_t count) {
+        T *q = reinterpret_cast<T *>(base + used);
+        used += (count * sizeof(T) + 255) / 256 * 256;
+        return q;
+    }
+};
+
+struct NoAllocTables {
+    cx<double> *tw[2], *wk[2], *qk[2];
+    int N[2], kind[2];
+    double *lamx, *lamy;
+    unsigned char *zx, *zy;
+    int Lx, Ly;
+    EigGeom eg;
+};
+
+// One axis's FFT tables on the device (the layout upload_tables writes on the
+// host: per-stage twiddles, real-FFT split and DCT quarter-wave factors), from
+// sincospi of the exact rational angle.
+__device__ void fill_axis_tables(cx<double> *tw, cx<double> *wk, cx<double> *qk, int N, int kind, int tid, int nt) {
+    const int M = (kind == AX_C2C) ? N : N / 2;
+    for (int e = tid; e < M; e += nt) {
+        cx<double> v{1.0, 0.0};  // entry M-1 is unused
+        int LS = 1;
+        while (LS < M) {
+            const int R = pick_radix(M / LS);
+            if (e < LS - 1 + (R - 1) * LS) {
+                const int q = e - (LS - 1), r = q / LS + 1, j = q - (r - 1) * LS;
+                double sv, cv;
+                sincospi(-2.0 * (double)(j * r) / (double)(LS * R), &sv, &cv);
+                v = cx<double>{cv, sv};
+                break;
+            }
+            LS *= R;
+        }
+        tw[e] = v;
+    }
+    for (int k = tid; k <= M; k += nt) {
+        double sv, cv;
+        sincospi(-2.0 * (double)k / (double)N, &sv, &cv);
+        wk[k] = cx<double>{cv, sv};
+        sincospi(-(double)k / (2.0 * (double)N), &sv, &cv);
+        qk[k] = cx<double>{cv, sv};
+    }
+}
+
+__global__ void __launch_bounds__(256) k_noalloc_tables(NoAllocTables t) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    for (int a = 0; a < 2; ++a) fill_axis_tables(t.tw[a], t.wk[a], t.qk[a], t.N[a], t.kind[a], tid, nt);
+    for (int kx = tid; kx < t.Lx; kx += nt) eig_col(t.eg, kx, t.lamx[kx], t.zx[kx]);
+    for (int jp = tid; jp < t.Ly; jp += nt) eig_row(t.eg, jp, t.lamy[jp], t.zy[jp]);
+}
+}  // namespace
+
+extern "C" int ppf_solve_noalloc(const ppf_desc *desc, const double *prhs, double *P, const double *dmx1,
+                                 const double *dmx2, const double *dmx3, const double *dx3, void *stream) {
+    if (!desc || !prhs || !P || !dmx1 || !dmx2 || !dmx3 || !dx3) return PPF_ERR_INVALID_ARG;
+    int rc = desc_check(*desc);
+    if (rc) return rc;
+    if (desc->precision != PPF_PREC_FP64) return PPF_ERR_UNSUPPORTED;
+    ppf_plan p;  // geometry and device pointers only; owns nothing
+    plan_geometry(*desc, &p);
+    const int Mx = (p.xkind == AX_C2C) ? desc->n1m : desc->n1m / 2;
+    const int My = (p.ykind == AX_C2C) ? desc->n2m : desc->n2m / 2;
+    const int n3m = desc->n3m;
+    const bool need_cs = n3m > 2048;  // the one-thread-per-line z kernel's multipliers
+    // bytes the slices below take (each rounded up to 256)
+    auto sz = [](size_t bytes) { return (bytes + 255) / 256 * 256; };
+    const size_t need = sz(sizeof(double) * (size_t)p.Lxs * p.Ly * p.Lz) + sz(sizeof(cx<double>) * Mx) +
+                        2 * sz(sizeof(cx<double>) * (Mx + 1)) + sz(sizeof(cx<double>) * My) +
+                        2 * sz(sizeof(cx<double>) * (My + 1)) + sz(sizeof(double) * (2 * (size_t)n3m + 4)) +
+                        sz(sizeof(double) * p.Lx) + sz(sizeof(double) * p.Ly) + sz(p.Lx) + sz(p.Ly) +
+                        sz(sizeof(double)) +
+                        (need_cs ? sz(sizeof(double) * (size_t)p.Lx * p.znjr * n3m) : 0);
+    if (need > kScratchBytes) return PPF_ERR_SCRATCH;
+    double *base = nullptr;
+    if ((rc = scratch_base(&base))) return rc;
+    ScratchCarver c{reinterpret_cast<char *>(base)};
+    p.W = c.take<double>((size_t)p.Lxs * p.Ly * p.Lz);
+    p.tx64.tw = c.take<cx<double>>(Mx);
+    p.tx64.wk = c.take<cx<double>>(Mx + 1);
+    p.tx64.qk = c.take<cx<double>>(Mx + 1);
+    p.ty64.tw = c.take<cx<double>>(My);
+    p.ty64.wk = c.take<cx<double>>(My + 1);
+    p.ty64.qk = c.take<cx<double>>(My + 1);
+    p.zcoef = c.take<double>(2 * (size_t)n3m + 4);
+    p.lamx = c.take<double>(p.Lx);
+    p.lamy = c.take<double>(p.Ly);
+    p.zx = c.take<unsigned char>(p.Lx);
+    p.zy = c.take<unsigned char>(p.Ly);
+    p.mean = c.take<double>(1);
+    if (need_cs) {
+        p.cs_elems = (size_t)p.Lx * p.znjr * n3m;
+        p.cs = c.take<double>(p.cs_elems);
+    }
+    if (c.used != need) return PPF_ERR_UNSUPPORTED;  // the size check and the slices disagree
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    NoAllocTables t{};
+    t.tw[0] = p.tx64.tw; t.wk[0] = p.tx64.wk; t.qk[0] = p.tx64.qk; t.N[0] = desc->n1m; t.kind[0] = p.xkind;
+    t.tw[1] = p.ty64.tw; t.wk[1] = p.ty64.wk; t.qk[1] = p.ty64.qk; t.N[1] = desc->n2m; t.kind[1] = p.ykind;
+    t.lamx = p.lamx; t.lamy = p.lamy; t.zx = p.zx; t.zy = p.zy;
+    t.Lx = p.Lx; t.Ly = p.Ly;
+    t.eg = eig_geom(&p);
+    hipLaunchKernelGGL(k_noalloc_tables, dim3(4), dim3(256), 0, s, t);
+    PPF_HIP(hipGetLastError());
+    return run_passes<double>(&p, p.W, p.tx64, p.ty64, prhs, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, nullptr,
+                              nullptr);
+}
+
+extern "C" {
 
 // ---- device memory shim ------------------------------------------------------
 int ppf_malloc(void **ptr, size_t bytes) {
