@@ -124,15 +124,16 @@ int ppf_solve_sweeps(ppf_plan *plan, const double *prhs, double *p, const double
 
 /* A solve that never synchronises the host, for callers that capture it in a
  * hipGraph or replay it on new right-hand sides.  FP32_IR plans: the FP32
- * solve, min_sweeps corrections, then -- up to max_sweeps -- an FP64 residual
- * test on the device before each further correction; once the relative
- * residual is <= tol every later correction and test is a no-op (tol < 0:
- * all max_sweeps corrections run).  Ghost layers are re-derived at the end.
+ * solve, min_sweeps corrections, then an FP64 residual test on the device
+ * before each of up to extra_sweeps further corrections (extra_sweeps < 0:
+ * up to max_sweeps corrections in all); once the relative residual is <= tol
+ * every later correction and test is a no-op, though still a kernel launch
+ * (tol < 0: all of them run).  Ghost layers are re-derived at the end.
  * Only kernel launches on `stream` (no memset, no copy).  FP64 plans: a plain
  * solve.  No counterpart in the reference (its solve is FP64 only and
  * synchronous, fft_3d_poisson.f90:353-626); used by the batched solve. */
 int ppf_solve_async(ppf_plan *plan, const double *prhs, double *p, const double *dmx1, const double *dmx2,
-                    const double *dmx3, const double *dx3, int min_sweeps, void *stream);
+                    const double *dmx3, const double *dx3, int min_sweeps, int extra_sweeps, void *stream);
 
 /* After a ppf_solve_async on `stream`: synchronises the stream and reports
  * the corrections it ran and the last relative residual its device-side test
@@ -175,8 +176,8 @@ typedef struct ppf_batch ppf_batch;
  * The first ppf_batch_solve solves every problem directly (FP32_IR problems
  * with their convergence test), records the sweeps each needed, and captures
  * the whole batch -- every problem as its own branch, FP32_IR problems as
- * ppf_solve_async with those sweep counts as min_sweeps -- into one hipGraph;
- * later calls replay the graph. */
+ * ppf_solve_async with those sweep counts as min_sweeps and one guarded extra
+ * sweep -- into one hipGraph; later calls replay the graph. */
 int ppf_batch_create(int nprob, const ppf_desc *descs, const ppf_batch_io *io, int flags, ppf_batch **batch);
 
 /* Solves every problem of the batch on `stream`.  stats (nprob entries or

@@ -94,7 +94,7 @@ def lib() -> C.CDLL:
     L.ppf_set_device.argtypes = [I]
     L.ppf_copy_timed.argtypes = [vp, vp, sz, I, vp, C.POINTER(C.c_float)]
     L.ppf_solve_sweeps.argtypes = [vp, vp, vp, vp, vp, vp, vp, I, vp]
-    L.ppf_solve_async.argtypes = [vp, vp, vp, vp, vp, vp, vp, I, vp]
+    L.ppf_solve_async.argtypes = [vp, vp, vp, vp, vp, vp, vp, I, I, vp]
     L.ppf_solve_noalloc.argtypes = [C.POINTER(Desc), vp, vp, vp, vp, vp, vp, vp]
     L.ppf_solve_async_result.argtypes = [vp, vp, C.POINTER(Stats)]
     L.ppf_plan_tune.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(I)]
@@ -259,10 +259,11 @@ class FFTPoisson:
                                       int(sweeps), stream), "ppf_solve_sweeps")
 
     def solve_async(self, prhs: DeviceArray, p: DeviceArray, dmx1: DeviceArray, dmx2: DeviceArray,
-                    dmx3: DeviceArray, dx3: DeviceArray, min_sweeps: int = 0, stream=None) -> None:
+                    dmx3: DeviceArray, dx3: DeviceArray, min_sweeps: int = 0, extra_sweeps: int = -1,
+                    stream=None) -> None:
         """Solve with the FP32_IR convergence test on the device (no host synchronisation)."""
         _check(lib().ppf_solve_async(self.handle, prhs.ptr, p.ptr, dmx1.ptr, dmx2.ptr, dmx3.ptr, dx3.ptr,
-                                     int(min_sweeps), stream), "ppf_solve_async")
+                                     int(min_sweeps), int(extra_sweeps), stream), "ppf_solve_async")
 
     def async_result(self, stream=None, allow_not_converged: bool = False) -> Stats:
         """Sweeps and residual of the last solve_async (synchronises)."""

@@ -9,8 +9,9 @@
 // run concurrently on the device, so small problems that cannot fill 256 CUs
 // on their own overlap instead of queueing, and the whole batch costs one
 // graph launch on the host.  An FP32_IR branch runs the learned sweeps and
-// then tests its residual on the device: a new right-hand side that needs
-// more sweeps gets them (up to max_sweeps) within the same replay.
+// then tests its residual on the device: a new right-hand side that needs one
+// more sweep gets it within the same replay, and one that needs more is
+// reported (PPF_ERR_NOT_CONVERGED from a solve with stats).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,6 +24,9 @@
 
 namespace {
 constexpr int kBranchStreams = 8;  // capture streams (default priority)
+// guarded refinement sweeps an FP32_IR branch carries beyond the learned
+// count: each is ~9 graph nodes per problem even when its gate is closed
+constexpr int kGuardedSweeps = 1;
 }
 
 struct ppf_batch {
@@ -68,7 +72,7 @@ static int batch_capture(ppf_batch *b) {
     for (int i = 0; i < b->n && !rc; ++i) {
         const ppf_batch_io &q = b->io[i];
         rc = ppf_solve_async(b->plan[i], q.prhs, q.p, q.dmx1, q.dmx2, q.dmx3, q.dx3, b->fp32[i] ? b->sweeps[i] : 0,
-                             b->streams[i % ns]);
+                             kGuardedSweeps, b->streams[i % ns]);
     }
     for (int s = 0; s < ns && !rc; ++s) {
         if (hipEventRecord(b->events[1 + s], b->streams[s]) != hipSuccess ||
@@ -177,7 +181,7 @@ int ppf_batch_solve(ppf_batch *b, void *stream, ppf_stats *stats) {
         for (int i = 0; i < b->n && !rc; ++i) {
             const ppf_batch_io &q = b->io[i];
             rc = ppf_solve_async(b->plan[i], q.prhs, q.p, q.dmx1, q.dmx2, q.dmx3, q.dx3, b->fp32[i] ? b->sweeps[i] : 0,
-                                 stream);
+                                 kGuardedSweeps, stream);
         }
         if (rc) return rc;
         replayed = true;

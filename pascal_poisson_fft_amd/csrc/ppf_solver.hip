@@ -396,15 +396,18 @@ static int solve_fp32_ir(ppf_plan *p, const double *f, double *P, const double *
 
 // FP32_IR solve with the convergence test on the device, for callers that
 // cannot synchronise (graph capture, graph replays of new right-hand sides).
-// min_sweeps corrections always run; then, up to max_sweeps, a residual test
-// (k_residual + k_gate_check) decides on the device whether the next gated
-// sweep runs.  Once the residual is <= tol every later gated kernel returns at
-// once.  Only kernel launches: no host synchronisation, no memset.
+// min_sweeps corrections always run; then a residual test (k_residual +
+// k_gate_check) decides on the device whether each of up to extra_sweeps
+// gated sweeps runs (extra_sweeps < 0: up to max_sweeps in all).  Once the
+// residual is <= tol every later gated kernel returns at once -- but is still
+// a launch, which is why a captured graph keeps extra_sweeps small.  Only
+// kernel launches: no host synchronisation, no memset.
 static int solve_fp32_async(ppf_plan *p, const double *f, double *P, const double *dmx1, const double *dmx2,
-                            const double *dmx3, const double *dx3, int min_sweeps, hipStream_t s) {
+                            const double *dmx3, const double *dx3, int min_sweeps, int extra_sweeps, hipStream_t s) {
     const double tol = p->d.tol == 0.0 ? 1e-10 : p->d.tol;  // tol < 0 never closes the gate
     const int maxs = p->d.max_sweeps > 0 ? p->d.max_sweeps : 8;
     const int nmin = std::min(min_sweeps, maxs);
+    const int nmax = extra_sweeps < 0 ? maxs : std::min(maxs, nmin + extra_sweeps);
     struct GateReset {  // p->gate is back to nullptr on every return path
         ppf_plan *p;
         ~GateReset() { p->gate = nullptr; }
@@ -423,7 +426,7 @@ static int solve_fp32_async(ppf_plan *p, const double *f, double *P, const doubl
         p->gate = p->dgate;  // open at the first test
         if ((rc = launch_residual(p, f, P, s))) return rc;
         hipLaunchKernelGGL(k_gate_check, dim3(1), dim3(1), 0, s, p->red, tol, i > nmin ? 1 : 0, p->dgate, p->dres);
-        if (i >= maxs) break;
+        if (i >= nmax) break;
         if ((rc = sweep32(p, f, P, xr, dmx1, dmx2, dmx3, dx3, s))) return rc;
     }
     p->gate = nullptr;
@@ -458,11 +461,11 @@ int ppf_solve_sweeps(ppf_plan *p, const double *prhs, double *P, const double *d
 }
 
 int ppf_solve_async(ppf_plan *p, const double *prhs, double *P, const double *dmx1, const double *dmx2,
-                    const double *dmx3, const double *dx3, int min_sweeps, void *stream) {
+                    const double *dmx3, const double *dx3, int min_sweeps, int extra_sweeps, void *stream) {
     if (!p || !prhs || !P || !dmx1 || !dmx2 || !dmx3 || !dx3 || min_sweeps < 0) return PPF_ERR_INVALID_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (p->d.precision == PPF_PREC_FP32_IR)
-        return solve_fp32_async(p, prhs, P, dmx1, dmx2, dmx3, dx3, min_sweeps, s);
+        return solve_fp32_async(p, prhs, P, dmx1, dmx2, dmx3, dx3, min_sweeps, extra_sweeps, s);
     return run_passes<double>(p, p->W, p->tx64, p->ty64, prhs, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, nullptr,
                               nullptr);
 }

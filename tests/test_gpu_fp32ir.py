@@ -140,6 +140,13 @@ def test_fp32_ir_device_side_convergence(pbc):
         plan.solve_async(dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3, min_sweeps=st.sweeps + 1)
         st2 = plan.async_result()
         assert st2.sweeps == st.sweeps + 1 and st2.rel_residual <= TOL
+        # no guarded sweeps allowed: the FP32 solve alone, reported as a miss
+        plan.solve_async(dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3, min_sweeps=0, extra_sweeps=0)
+        with pytest.raises(api.PPFError) as ei:
+            plan.async_result()
+        assert ei.value.code == api.PPF_ERR_NOT_CONVERGED
+        st3 = plan.async_result(allow_not_converged=True)
+        assert st3.sweeps == 0 and st3.rel_residual > TOL
     finally:
         dev.free()
         plan.clean()
