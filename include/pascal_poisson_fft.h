@@ -151,8 +151,9 @@ int ppf_solve_async_result(ppf_plan *plan, void *stream, ppf_stats *stats);
 int ppf_plan_tune(ppf_plan *plan, const double *prhs, double *p, const double *dmx1, const double *dmx2,
                   const double *dmx3, const double *dx3, void *stream, int *zvariant);
 
-/* Pins the z-pass kernel shape (an id returned by ppf_plan_tune; 0 = the
- * default chosen by n3m). */
+/* Pins the z-pass kernel shape (an id returned by ppf_plan_tune, 1..17; 0 =
+ * the default chosen by n3m).  A shape that does not fit the plan's z extent
+ * makes later solves return PPF_ERR_UNSUPPORTED. */
 int ppf_plan_set_zvariant(ppf_plan *plan, int zvariant);
 
 /* ---- batched solves of independent problems ----------------------------- */

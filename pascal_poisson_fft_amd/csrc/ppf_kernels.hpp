@@ -131,10 +131,6 @@ __device__ __forceinline__ int fdiv_small(int a, int b, float rb) {
     return q;
 }
 
-// Residual input of the refinement form of pass 1: r = f - L x on interior
-// cells, x the ghosted FP64 iterate.  L is the operator the solve inverts:
-// uniform x/y spacing with zero-flux walls (the DCT eigenvalues) or wrap, and
-// the stretched z rows zam/zap of fft_3d_poisson.f90:1143-1146.
 // z row k (0-based) of the stretched operator, fft_3d_poisson.f90:1143-1146:
 // am couples to row k-1 (0 on the bottom row), ap to row k+1 (0 on the top).
 __device__ __forceinline__ void z_row(const double *__restrict__ dx3, const double *__restrict__ dmx3, int n3m, int k,
@@ -166,46 +162,19 @@ __device__ __forceinline__ void fill_z_rows(const ZRows &zr, int tid, int nt) {
     if (tid == 0) z_wall_bc(zr.dmx3, zr.n3m, zr.zap + zr.n3m);
 }
 
+// Inputs of the FP64 residual r = f - L x on interior cells, x the ghosted
+// FP64 iterate.  L is the operator the solve inverts: uniform x/y spacing with
+// zero-flux walls (the DCT eigenvalues) or wrap, and the stretched z rows
+// zam/zap of fft_3d_poisson.f90:1143-1146.
 struct XRes {
     const double *P;          // (n1m+2)(n2m+2)(n3m+2) iterate
     const double *zam, *zap;  // z rows (zero at the walls)
     double *partial;          // per-workgroup sums of r^2, f^2 and x (3 per workgroup)
     double ih1, ih2;          // 1/h^2
     int pbc1, pbc2;
+    float *rout;              // nullptr, or W32: r stored in FP32 where pass 1 reads line (j,k)
+    int Ly, Lxs;              // W32 row count per plane and row stride
 };
-
-// r at interior cell (n, j, k) (0-based); fv receives f there.  Neighbour
-// addresses are selected, never loaded values, so the seven loads issue
-// together.
-__device__ __forceinline__ double residual_at(const double *__restrict__ f, const XRes &r, int n, int j, int k, int N,
-                                              int n2m, double &fv, double &xv) {
-    const size_t L1 = (size_t)N + 2;
-    const size_t L2 = (size_t)n2m + 2;
-    const size_t pc = ((size_t)(k + 1) * L2 + (size_t)(j + 1)) * L1 + (size_t)(n + 1);
-    const size_t pl = (size_t)L1 * L2;
-    size_t ixm = pc - 1, ixp = pc + 1, iym = pc - L1, iyp = pc + L1;
-    if (r.pbc1) {
-        ixm = n == 0 ? pc + (N - 1) : ixm;
-        ixp = n == N - 1 ? pc - (N - 1) : ixp;
-    } else {
-        ixm = n == 0 ? pc : ixm;
-        ixp = n == N - 1 ? pc : ixp;
-    }
-    if (r.pbc2) {
-        iym = j == 0 ? pc + (size_t)(n2m - 1) * L1 : iym;
-        iyp = j == n2m - 1 ? pc - (size_t)(n2m - 1) * L1 : iyp;
-    } else {
-        iym = j == 0 ? pc : iym;
-        iyp = j == n2m - 1 ? pc : iyp;
-    }
-    const double c = r.P[pc], xm = r.P[ixm], xp = r.P[ixp], ym = r.P[iym], yp = r.P[iyp];
-    const double zm = r.P[pc - pl], zp = r.P[pc + pl];  // ghost planes at the walls (finite, weight 0)
-    const double am = r.zam[k], ap = r.zap[k];
-    fv = f[((size_t)k * n2m + j) * N + n];
-    xv = c;
-    const double lp = (xp - 2.0 * c + xm) * r.ih1 + (yp - 2.0 * c + ym) * r.ih2 + am * zm - (am + ap) * c + ap * zp;
-    return fv - lp;
-}
 
 // Sum of v over a workgroup of NT <= 256 threads (any count); thread 0 gets
 // the total.  red holds NT doubles of LDS.  Fixed order: deterministic.
@@ -230,25 +199,13 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 // tile has been scattered into LDS, so they are in flight while the current
 // tile is transformed and stored (barriers wait on LDS only).
 //
-// RES: the refinement form -- the transformed line is the FP64 residual
-// r = f - L x (in = f, xr.P = x) rather than in itself, and the workgroup's
-// sums of r^2, f^2 and x go to xr.partial.
+// RES: the refinement form -- the line is the residual r = f - L x that
+// k_residual stored in FP32 in this line's own W row (in is unused), and the
+// transform overwrites it there.
 template <typename T, int N, bool PER, bool RES = false>
-// The refinement forms carry the residual stencil / the prefetched iterate:
-// uncapped they take one wave per SIMD (256 VGPRs + AGPRs).  From lines of
-// PPF_XIR_CAP_N points on they are capped at two waves: that spills 124 and
-// 24 bytes per lane and measured 0.36 ms faster per 512^3 refinement solve,
-// while on the 32..256-point lines of the 96-problem batch the spills cost
-// more than the occupancy gains (FP32+IR batch 12.3 vs 11.3 ms).
-#ifndef PPF_XRES_WPE
-#define PPF_XRES_WPE 2
-#endif
-#ifndef PPF_XIR_CAP_N
-#define PPF_XIR_CAP_N 512
-#endif
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RES ? (N >= PPF_XIR_CAP_N ? PPF_XRES_WPE : 1) : XGeom<T, N, PER>::WPE)))
-k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, int Ly,
-       const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk, XRes xr,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XGeom<T, N, PER>::WPE)))
+k_xfwd(const double *__restrict__ in, T *W, int nlines, int n2m, int Ly,
+       const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk,
        ZRows zr, const int *__restrict__ gate) {
     if (gated_off(gate)) return;
     using X = XGeom<T, N, PER>;
@@ -266,7 +223,6 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
     const int ntiles = (nlines + LPB - 1) / LPB;
     const float rn2m = 1.0f / (float)n2m;
     D2 pre[EPT];
-    double sr = 0.0, sf = 0.0, sx = 0.0;
     // unconditional 16-byte loads of sample pairs from clamped addresses
     // (lines past the end are transformed but never stored); nothing
     // consumes them before the next tile, so they stay in flight
@@ -279,19 +235,11 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
             const int line0 = tile * LPB + li;
             const int line = line0 < nlines ? line0 : nlines - 1;
             if constexpr (RES) {
-                const int k = fdiv_small(line, n2m, rn2m), j = line - k * n2m;
-                double fv0, xv0, fv1, xv1;
-                const double r0 = residual_at(in, xr, n, j, k, N, n2m, fv0, xv0);
-                const double r1 = residual_at(in, xr, n + 1, j, k, N, n2m, fv1, xv1);
-                pre[q] = D2{r0, r1};
-                if (e0 < LPB * H && line0 < nlines) {
-                    sr += r0 * r0 + r1 * r1;
-                    sf += fv0 * fv0 + fv1 * fv1;
-                    sx += xv0 + xv1;
-                }
-                // one pair's stencil loads at a time: r is needed at once, so
-                // keeping every pair's loads in flight only costs registers
-                __builtin_amdgcn_sched_barrier(0);
+                // W row of line (j,k): line + Ly + 1 + 2k (as stored below)
+                const int k = fdiv_small(line, n2m, rn2m);
+                const V2 v = ld_stream(reinterpret_cast<const V2 *>(
+                    &W[((size_t)line + (size_t)(Ly + 1) + 2 * (size_t)k) * LXS + n]));
+                pre[q] = D2{(double)v.x, (double)v.y};
             } else {
                 pre[q] = ld_stream(reinterpret_cast<const D2 *>(&in[(size_t)line * N + n]));
             }
@@ -346,17 +294,6 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
         }
         if (PER && tid < LPB) store_pair(tid, HX - 1);
     }
-    if constexpr (RES) {
-        __shared__ double red[NT];
-        const double tr = block_sum<NT>(sr, red);
-        const double tf = block_sum<NT>(sf, red);
-        const double tx = block_sum<NT>(sx, red);
-        if (tid == 0) {
-            xr.partial[3 * blockIdx.x] = tr;
-            xr.partial[3 * blockIdx.x + 1] = tf;
-            xr.partial[3 * blockIdx.x + 2] = tx;
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -366,10 +303,18 @@ k_xfwd(const double *__restrict__ in, T *__restrict__ W, int nlines, int n2m, in
 // mean(x), xsum[2] = sum of x's interior) instead of stored; P's rows are
 // prefetched with the W rows.
 // ---------------------------------------------------------------------------
-template <typename T, int N, bool PER, bool ACC = false>
+// The ACC form carries the prefetched iterate: uncapped it takes one wave per
+// SIMD (256 VGPRs + AGPRs).  From lines of PPF_XIR_CAP_N points on it is
+// capped at two waves (0.65 -> 0.47 ms at 512^3), while on the 32..256-point
+// lines of the 96-problem batch the spills cost more than the occupancy
+// gains.
 #ifndef PPF_XACC_WPE
 #define PPF_XACC_WPE 2
 #endif
+#ifndef PPF_XIR_CAP_N
+#define PPF_XIR_CAP_N 512
+#endif
+template <typename T, int N, bool PER, bool ACC = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ACC ? (N >= PPF_XIR_CAP_N ? PPF_XACC_WPE : 1) : XGeom<T, N, PER>::WPE)))
 k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk,
@@ -1461,12 +1406,15 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
 
 // ---------------------------------------------------------------------------
 // FP64 residual r = f - L p on interior cells of the ghosted P (the operator
-// of residual_at), with per-workgroup sums of r^2, f^2 and p (deterministic
-// two-level reduction with k_reduce3).  A tile is 256 consecutive x cells of
-// one row j over KZ consecutive planes; each thread walks its column up the
-// planes keeping p(k-1), p(k), p(k+1) in registers, so each plane of P is
-// loaded once per column (its x/y neighbours come from the caches).
-// Workgroups stride over the tiles.
+// of XRes), with per-workgroup sums of r^2, f^2 and p (deterministic
+// two-level reduction with k_reduce3); with xr.rout set, r is also stored in
+// FP32 in the W row pass 1 transforms next (the refinement sweep's input).  A
+// tile is 256 consecutive x cells of one row j over KZ consecutive planes;
+// each thread walks its column up the planes keeping p(k-1), p(k), p(k+1) in
+// registers, so each plane of P is loaded once per column (its x/y
+// neighbours come from the caches).  Workgroups stride over the tiles.
+// (Giving each XCD consecutive tiles, so that y neighbours meet in one L2,
+// halved the fetched bytes and still measured slower: 0.70 vs 0.65 ms.)
 // ---------------------------------------------------------------------------
 template <int KZ>
 __global__ void __launch_bounds__(256)
@@ -1523,6 +1471,8 @@ k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m, con
                 sr += r * r;
                 sf += fv * fv;
                 sx += c;
+                if (xr.rout)
+                    st_stream(&xr.rout[((size_t)(k + 1) * (size_t)xr.Ly + (size_t)(j + 1)) * (size_t)xr.Lxs + i], (float)r);
             }
             zm = c;
             c = zp;

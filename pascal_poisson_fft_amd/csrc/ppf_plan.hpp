@@ -97,18 +97,23 @@ struct ppf_plan {
 
 namespace ppf {
 
+// z-pass shapes: 1-5 and 12-16 partitioned (k_zpart), 6 one thread per line
+// (k_zthomas), 7-11 and 17 pipelined (k_zpipe); 0 = chosen by n3m.
+constexpr int kZVariants = 17;
+
 // The z row table of a plan (filled by pass 1 when passed to dispatch_xfwd).
 inline ZRows z_rows(const ppf_plan *p, const double *dx3, const double *dmx3) {
     return ZRows{dx3, dmx3, p->zcoef, p->zcoef + p->d.n3m, p->d.n3m};
 }
 
 // Pass launchers (T = double or float).  nplanes < 0: the whole field.
-// xr != nullptr (FP32 plans only): the residual form of pass 1; *grid gets
-// the workgroup count (partial sums).  zr.zam set: workgroup 0 of pass 1 also
-// fills the z row table.  acc (FP32 plans only): pass 5 adds to P.
+// res (FP32 plans only): the refinement form of pass 1, transforming the
+// residual k_residual stored in the W rows (in unused).  zr.zam set:
+// workgroup 0 of pass 1 also fills the z row table.  acc (FP32 plans only):
+// pass 5 adds to P.
 template <typename T>
-int dispatch_xfwd(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, const XRes *xr, hipStream_t s,
-                  unsigned *grid = nullptr, int nplanes = -1, ZRows zr = ZRows{});
+int dispatch_xfwd(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, bool res, hipStream_t s,
+                  int nplanes = -1, ZRows zr = ZRows{});
 template <typename T>
 int dispatch_xinv(const ppf_plan *p, const T *W, double *P, const AxisTables<T> &tb, const double *dmx1, bool acc,
                   hipStream_t s, int nplanes = -1);

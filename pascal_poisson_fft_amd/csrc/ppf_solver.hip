@@ -242,18 +242,16 @@ size_t ppf_plan_workspace_bytes(const ppf_plan *plan) { return plan ? plan->byte
 
 }  // extern "C"
 
-// The five passes of one solve of the plan's precision.  xr != nullptr: pass 1
-// transforms the residual f - L P (refinement form); acc: pass 5 adds to P.
-// ev (7 events or nullptr) brackets the passes for ppf_solve_timed.
+// The five passes of one solve of the plan's precision: in -> P.  ev (6 events
+// or nullptr) brackets the passes for ppf_solve_timed.
 template <typename T>
 static int run_passes(ppf_plan *p, T *W, const AxisTables<T> &tx, const AxisTables<T> &ty, const double *in,
-                      const XRes *xr, double *P, bool acc, const double *dmx1, const double *dmx2, const double *dmx3,
-                      const double *dx3, hipStream_t s, hipEvent_t *ev, unsigned *res_grid) {
+                      double *P, const double *dmx1, const double *dmx2, const double *dmx3, const double *dx3,
+                      hipStream_t s, hipEvent_t *ev) {
     int rc;
     if (ev) PPF_HIP(hipEventRecord(ev[0], s));
-    // pass 1 also fills the z row table of this solve (not in the residual
-    // form, which reads it: the table of the solve that made the iterate)
-    if ((rc = dispatch_xfwd<T>(p, in, W, tx, xr, s, res_grid, -1, xr ? ZRows{} : z_rows(p, dx3, dmx3)))) return rc;
+    // pass 1 also fills the z row table of this solve
+    if ((rc = dispatch_xfwd<T>(p, in, W, tx, false, s, -1, z_rows(p, dx3, dmx3)))) return rc;
     if (ev) PPF_HIP(hipEventRecord(ev[1], s));
     if ((rc = dispatch_y<T>(p, W, ty, false, dmx2, s))) return rc;
     if (ev) PPF_HIP(hipEventRecord(ev[2], s));
@@ -261,7 +259,7 @@ static int run_passes(ppf_plan *p, T *W, const AxisTables<T> &tx, const AxisTabl
     if (ev) PPF_HIP(hipEventRecord(ev[3], s));
     if ((rc = dispatch_y<T>(p, W, ty, true, dmx2, s))) return rc;
     if (ev) PPF_HIP(hipEventRecord(ev[4], s));
-    if ((rc = dispatch_xinv<T>(p, W, P, tx, dmx1, acc, s))) return rc;
+    if ((rc = dispatch_xinv<T>(p, W, P, tx, dmx1, false, s))) return rc;
     if (ev) PPF_HIP(hipEventRecord(ev[5], s));
     PPF_HIP(hipGetLastError());
     return PPF_OK;
@@ -282,10 +280,16 @@ static XRes make_xres(const ppf_plan *p, const double *P) {
 }
 
 // FP64 residual of P against f into p->red (sum r^2, sum f^2, sum p), from
-// the z row table already in p->zcoef; gated by p->gate.
-static int launch_residual(ppf_plan *p, const double *f, const double *P, hipStream_t s) {
+// the z row table already in p->zcoef; gated by p->gate.  out (FP32 plans):
+// r is also stored in W32, where the refinement form of pass 1 reads it.
+static int launch_residual(ppf_plan *p, const double *f, const double *P, hipStream_t s, bool out = false) {
     const int n3m = p->d.n3m;
-    const XRes xr = make_xres(p, P);
+    XRes xr = make_xres(p, P);
+    if (out) {
+        xr.rout = p->W32;
+        xr.Ly = p->Ly;
+        xr.Lxs = p->Lxs;
+    }
     constexpr int KZ = 32;  // planes per tile (4..32 and 8 or 16 workgroups per CU measured alike)
     const long ntiles = (long)((p->d.n1m + 255) / 256) * p->d.n2m * ((n3m + KZ - 1) / KZ);
     const int g = (int)std::min<long>(std::min<long>(p->npartial, 8L * num_cus()), ntiles);
@@ -313,16 +317,14 @@ static int refresh_ghosts(const ppf_plan *p, double *P, const double *dmx1, cons
     return PPF_OK;
 }
 
-// One refinement sweep of an FP32_IR plan: pass 1 transforms r = f - L x
-// (FP64, with its sums into p->red), passes 2-4 in FP32, pass 5 adds the
-// correction to x.  Gated by p->gate.
-static int sweep32(ppf_plan *p, const double *f, double *P, const XRes &xr, const double *dmx1, const double *dmx2,
-                   const double *dmx3, const double *dx3, hipStream_t s) {
-    unsigned g = 0;
+// The correction of one refinement sweep of an FP32_IR plan: pass 1
+// transforms the residual launch_residual(out) left in W32, passes 2-4 in
+// FP32, pass 5 adds the correction to P.  Gated by p->gate.
+static int correct32(ppf_plan *p, double *P, const double *dmx1, const double *dmx2, const double *dmx3,
+                     const double *dx3, hipStream_t s) {
     int rc;
-    if ((rc = dispatch_xfwd<float>(p, f, p->W32, p->tx32, &xr, s, &g))) return rc;
-    hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, s, p->partial, (int)g, p->red, p->gate);
-    if ((rc = dispatch_y<float>(p, p->W32, p->ty32, false, dmx2, s)) || (rc = launch_z<float>(p, p->W32, dx3, dmx3, s)) ||
+    if ((rc = dispatch_xfwd<float>(p, nullptr, p->W32, p->tx32, true, s)) ||
+        (rc = dispatch_y<float>(p, p->W32, p->ty32, false, dmx2, s)) || (rc = launch_z<float>(p, p->W32, dx3, dmx3, s)) ||
         (rc = dispatch_y<float>(p, p->W32, p->ty32, true, dmx2, s)) ||
         (rc = dispatch_xinv<float>(p, p->W32, P, p->tx32, dmx1, true, s)))
         return rc;
@@ -344,46 +346,32 @@ static int solve_fp32_ir(ppf_plan *p, const double *f, double *P, const double *
     const double tol = fixed >= 0 ? -1.0 : (p->d.tol == 0.0 ? 1e-10 : p->d.tol);
     const int maxs = fixed >= 0 ? fixed : (p->d.max_sweeps > 0 ? p->d.max_sweeps : 8);
     int rc;
-    if ((rc = run_passes<float>(p, p->W32, p->tx32, p->ty32, f, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, nullptr,
-                                nullptr)))
-        return rc;
-    const XRes xr = make_xres(p, P);
+    if ((rc = run_passes<float>(p, p->W32, p->tx32, p->ty32, f, P, dmx1, dmx2, dmx3, dx3, s, nullptr))) return rc;
+    // the residuals below use the z row table pass 1 of this solve filled
     int sweeps = 0;
     double rel = -1.0;
     bool converged = false;
     if (tol < 0) {
         for (; sweeps < maxs; ++sweeps)
-            if ((rc = sweep32(p, f, P, xr, dmx1, dmx2, dmx3, dx3, s))) return rc;
+            if ((rc = launch_residual(p, f, P, s, true)) || (rc = correct32(p, P, dmx1, dmx2, dmx3, dx3, s))) return rc;
         if (sweeps > 0 && (rc = refresh_ghosts(p, P, dmx1, dmx2, dmx3, s))) return rc;
         if (stats) {
             if ((rc = ppf_residual(p, f, P, dmx3, dx3, s, &rel))) return rc;
         }
         converged = true;
     } else {
-        // The residual form of pass 1 computes ||r|| and transforms r for the
-        // next sweep; after a sweep the plain residual kernel (about half its
-        // time) checks convergence, so a converged solve does not pay for a
-        // transform it discards.  Both evaluate the same FP64 operator.
+        // Each test computes the FP64 residual and stores it in FP32 where
+        // pass 1 of the next correction reads it; the last test's store is
+        // the only one unused (4 of its 20 bytes per cell).
         for (;;) {
-            // residual of the current iterate, transformed by pass 1
-            unsigned g = 0;
-            if ((rc = dispatch_xfwd<float>(p, f, p->W32, p->tx32, &xr, s, &g))) return rc;
-            hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, s, p->partial, (int)g, p->red, p->gate);
+            if ((rc = launch_residual(p, f, P, s, sweeps < maxs))) return rc;
             PPF_HIP(hipMemcpyAsync(p->hred, p->red, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
             PPF_HIP(hipStreamSynchronize(s));
             rel = p->hred[1] > 0 ? std::sqrt(p->hred[0] / p->hred[1]) : std::sqrt(p->hred[0]);
             if (rel <= tol) { converged = true; break; }
             if (sweeps == maxs || !(rel == rel)) break;
-            if ((rc = dispatch_y<float>(p, p->W32, p->ty32, false, dmx2, s)) ||
-                (rc = launch_z<float>(p, p->W32, dx3, dmx3, s)) ||
-                (rc = dispatch_y<float>(p, p->W32, p->ty32, true, dmx2, s)) ||
-                (rc = dispatch_xinv<float>(p, p->W32, P, p->tx32, dmx1, true, s)))
-                return rc;
-            PPF_HIP(hipGetLastError());
+            if ((rc = correct32(p, P, dmx1, dmx2, dmx3, dx3, s))) return rc;
             ++sweeps;
-            if ((rc = ppf_residual(p, f, P, dmx3, dx3, s, &rel))) return rc;
-            if (rel <= tol) { converged = true; break; }
-            if (sweeps == maxs || !(rel == rel)) break;
         }
         if (sweeps > 0 && (rc = refresh_ghosts(p, P, dmx1, dmx2, dmx3, s))) return rc;
     }
@@ -398,10 +386,10 @@ static int solve_fp32_ir(ppf_plan *p, const double *f, double *P, const double *
 // cannot synchronise (graph capture, graph replays of new right-hand sides).
 // min_sweeps corrections always run; then a residual test (k_residual +
 // k_gate_check) decides on the device whether each of up to extra_sweeps
-// gated sweeps runs (extra_sweeps < 0: up to max_sweeps in all).  Once the
-// residual is <= tol every later gated kernel returns at once -- but is still
-// a launch, which is why a captured graph keeps extra_sweeps small.  Only
-// kernel launches: no host synchronisation, no memset.
+// gated corrections runs (extra_sweeps < 0: up to max_sweeps in all).  Once
+// the residual is <= tol every later gated kernel returns at once -- but is
+// still a launch, which is why a captured graph keeps extra_sweeps small.
+// Only kernel launches: no host synchronisation, no memset.
 static int solve_fp32_async(ppf_plan *p, const double *f, double *P, const double *dmx1, const double *dmx2,
                             const double *dmx3, const double *dx3, int min_sweeps, int extra_sweeps, hipStream_t s) {
     const double tol = p->d.tol == 0.0 ? 1e-10 : p->d.tol;  // tol < 0 never closes the gate
@@ -415,19 +403,16 @@ static int solve_fp32_async(ppf_plan *p, const double *f, double *P, const doubl
     p->gate = nullptr;
     hipLaunchKernelGGL(k_gate_init, dim3(1), dim3(1), 0, s, p->dgate, nmin);
     int rc;
-    if ((rc = run_passes<float>(p, p->W32, p->tx32, p->ty32, f, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, nullptr,
-                                nullptr)))
-        return rc;
-    const XRes xr = make_xres(p, P);
+    if ((rc = run_passes<float>(p, p->W32, p->tx32, p->ty32, f, P, dmx1, dmx2, dmx3, dx3, s, nullptr))) return rc;
+    // the z row table of this solve (filled by pass 1) serves the residuals
     for (int i = 0; i < nmin; ++i)
-        if ((rc = sweep32(p, f, P, xr, dmx1, dmx2, dmx3, dx3, s))) return rc;
-    // the z row table of this solve (filled by pass 1) serves the residual
+        if ((rc = launch_residual(p, f, P, s, true)) || (rc = correct32(p, P, dmx1, dmx2, dmx3, dx3, s))) return rc;
     for (int i = nmin;; ++i) {
         p->gate = p->dgate;  // open at the first test
-        if ((rc = launch_residual(p, f, P, s))) return rc;
+        if ((rc = launch_residual(p, f, P, s, i < nmax))) return rc;
         hipLaunchKernelGGL(k_gate_check, dim3(1), dim3(1), 0, s, p->red, tol, i > nmin ? 1 : 0, p->dgate, p->dres);
         if (i >= nmax) break;
-        if ((rc = sweep32(p, f, P, xr, dmx1, dmx2, dmx3, dx3, s))) return rc;
+        if ((rc = correct32(p, P, dmx1, dmx2, dmx3, dx3, s))) return rc;
     }
     p->gate = nullptr;
     if ((rc = refresh_ghosts(p, P, dmx1, dmx2, dmx3, s))) return rc;
@@ -443,8 +428,7 @@ int ppf_solve(ppf_plan *p, const double *prhs, double *P, const double *dmx1, co
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (p->d.precision == PPF_PREC_FP32_IR) return solve_fp32_ir(p, prhs, P, dmx1, dmx2, dmx3, dx3, s, stats);
     int rc;
-    if ((rc = run_passes<double>(p, p->W, p->tx64, p->ty64, prhs, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s,
-                                 nullptr, nullptr)))
+    if ((rc = run_passes<double>(p, p->W, p->tx64, p->ty64, prhs, P, dmx1, dmx2, dmx3, dx3, s, nullptr)))
         return rc;
     if (stats) { stats->sweeps = 0; stats->rel_residual = -1.0; }
     return PPF_OK;
@@ -456,8 +440,7 @@ int ppf_solve_sweeps(ppf_plan *p, const double *prhs, double *P, const double *d
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (p->d.precision == PPF_PREC_FP32_IR) return solve_fp32_ir(p, prhs, P, dmx1, dmx2, dmx3, dx3, s, nullptr, sweeps);
     if (sweeps != 0) return PPF_ERR_INVALID_ARG;
-    return run_passes<double>(p, p->W, p->tx64, p->ty64, prhs, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, nullptr,
-                              nullptr);
+    return run_passes<double>(p, p->W, p->tx64, p->ty64, prhs, P, dmx1, dmx2, dmx3, dx3, s, nullptr);
 }
 
 int ppf_solve_async(ppf_plan *p, const double *prhs, double *P, const double *dmx1, const double *dmx2,
@@ -466,8 +449,7 @@ int ppf_solve_async(ppf_plan *p, const double *prhs, double *P, const double *dm
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (p->d.precision == PPF_PREC_FP32_IR)
         return solve_fp32_async(p, prhs, P, dmx1, dmx2, dmx3, dx3, min_sweeps, extra_sweeps, s);
-    return run_passes<double>(p, p->W, p->tx64, p->ty64, prhs, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, nullptr,
-                              nullptr);
+    return run_passes<double>(p, p->W, p->tx64, p->ty64, prhs, P, dmx1, dmx2, dmx3, dx3, s, nullptr);
 }
 
 int ppf_solve_async_result(ppf_plan *p, void *stream, ppf_stats *stats) {
@@ -492,7 +474,7 @@ int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1
                   const double *dmx3, const double *dx3, void *stream, int *zvariant) {
     if (!p || !prhs || !P || !dmx1 || !dmx2 || !dmx3 || !dx3) return PPF_ERR_INVALID_ARG;
     // candidates: partitioned (K,S) shapes and the pipelined wavefront-PCR form
-    static const int cand[] = {3, 1, 2, 4, 5, 7, 8, 12, 13, 14};
+    static const int cand[] = {3, 1, 2, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17};
     const int keep = p->zvariant;
     int best = 0;
     float best_ms = 0.f;
@@ -515,7 +497,7 @@ int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1
 }
 
 int ppf_plan_set_zvariant(ppf_plan *p, int zvariant) {
-    if (!p || zvariant < 0 || zvariant > 14) return PPF_ERR_INVALID_ARG;
+    if (!p || zvariant < 0 || zvariant > kZVariants) return PPF_ERR_INVALID_ARG;
     if (zvariant == 6) {
         // the one-thread-per-line kernel keeps its forward multipliers in cs
         const size_t need = (size_t)p->Lx * p->znjr * p->d.n3m;
@@ -543,11 +525,9 @@ int ppf_solve_timed(ppf_plan *p, const double *prhs, double *P, const double *dm
         if (hipEventCreate(&ev[made]) != hipSuccess) { rc = PPF_ERR_HIP; break; }
     if (!rc) {
         if (p->d.precision == PPF_PREC_FP32_IR)
-            rc = run_passes<float>(p, p->W32, p->tx32, p->ty32, prhs, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, ev,
-                                   nullptr);
+            rc = run_passes<float>(p, p->W32, p->tx32, p->ty32, prhs, P, dmx1, dmx2, dmx3, dx3, s, ev);
         else
-            rc = run_passes<double>(p, p->W, p->tx64, p->ty64, prhs, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, ev,
-                                    nullptr);
+            rc = run_passes<double>(p, p->W, p->tx64, p->ty64, prhs, P, dmx1, dmx2, dmx3, dx3, s, ev);
     }
     if (!rc && hipEventSynchronize(ev[5]) != hipSuccess) rc = PPF_ERR_HIP;
     for (int i = 0; i < 5 && !rc; ++i) {
@@ -717,8 +697,7 @@ extern "C" int ppf_solve_noalloc(const ppf_desc *desc, const double *prhs, doubl
     t.eg = eig_geom(&p);
     hipLaunchKernelGGL(k_noalloc_tables, dim3(4), dim3(256), 0, s, t);
     PPF_HIP(hipGetLastError());
-    return run_passes<double>(&p, p.W, p.tx64, p.ty64, prhs, nullptr, P, false, dmx1, dmx2, dmx3, dx3, s, nullptr,
-                              nullptr);
+    return run_passes<double>(&p, p.W, p.tx64, p.ty64, prhs, P, dmx1, dmx2, dmx3, dx3, s, nullptr);
 }
 
 extern "C" {

@@ -64,13 +64,14 @@ int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hipStream
     // measured on MI355X at 512^3: (K=32, S=16) 0.54 ms, pipelined K=16 0.81 ms
     if (v == 0) v = (n3m <= 1024) ? 3 : 4;
     // the pipelined form writes wall ghost planes only
-    if (z.zper && v >= 7) v = (n3m <= 1024) ? 3 : 4;
+    if (z.zper && ((v >= 7 && v <= 11) || v == 17)) v = (n3m <= 1024) ? 3 : 4;
     switch (v) {
         case 7: pipe(integral_constant<int, 16>{}, integral_constant<int, 16>{}, integral_constant<int, 2>{}); break;
         case 8: pipe(integral_constant<int, 16>{}, integral_constant<int, 8>{}, integral_constant<int, 2>{}); break;
         case 9: pipe(integral_constant<int, 32>{}, integral_constant<int, 8>{}, integral_constant<int, 2>{}); break;
         case 10: pipe(integral_constant<int, 32>{}, integral_constant<int, 16>{}, integral_constant<int, 2>{}); break;
         case 11: pipe(integral_constant<int, 8>{}, integral_constant<int, 16>{}, integral_constant<int, 4>{}); break;
+        case 17: pipe(integral_constant<int, 24>{}, integral_constant<int, 16>{}, integral_constant<int, 2>{}); break;
         default: break;
     }
     switch (v) {
@@ -82,8 +83,14 @@ int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hipStream
         case 12: part(integral_constant<int, 24>{}, integral_constant<int, 16>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
         case 13: part(integral_constant<int, 24>{}, integral_constant<int, 8>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
         case 14: part(integral_constant<int, 48>{}, integral_constant<int, 8>{}, integral_constant<int, 512>{}, integral_constant<int, 1>{}); break;
+        // 32 columns: 128-byte row segments for FP32 workspaces
+        case 15: part(integral_constant<int, 32>{}, integral_constant<int, 32>{}, integral_constant<int, 512>{}, integral_constant<int, 2>{}); break;
+        case 16: part(integral_constant<int, 16>{}, integral_constant<int, 32>{}, integral_constant<int, 512>{}, integral_constant<int, 3>{}); break;
         default: break;
     }
+    // a pinned shape that does not fit the extent is reported, never replaced:
+    // only the default (v chosen above) and shape 6 fall to one thread per line
+    if (!done && p->zvariant != 0 && p->zvariant != 6) return PPF_ERR_UNSUPPORTED;
     if (!done) {
         const long ncol = (long)p->Lx * p->znjr;
         // the one-thread-per-line kernel keeps its forward multipliers in cs

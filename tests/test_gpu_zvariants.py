@@ -37,7 +37,7 @@ from pascal_poisson_fft_amd.grid import make_axis
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = list(range(1, 15))
+VARIANTS = list(range(1, 18))
 UNSUPPORTED = -5
 
 
