@@ -42,8 +42,10 @@ typedef struct ppf_plan ppf_plan;
 /* Problem description.  Replaces the scalar arguments of
  * fft_poisson_plan_cuda_create (fft_3d_poisson.f90:70-108): the reference
  * passes n = nm+1 and derives nm = n-1 (:103-104); here the interior counts nm
- * are given directly.  Extents must be even, 8 <= nm <= 1024 in x and y with
- * nm of the form 2^a or 3*2^a, and nm >= 2 in z. */
+ * are given directly.  Extents must be even (01_Fortran/examples/
+ * global.f90:114-120): 8 <= nm <= 1024 in x and y with nm = 2^a times 1, 3,
+ * 5, 7 or 15 (the lengths with compiled line transforms), 2 <= nm <= 4096 in
+ * z. */
 typedef struct ppf_desc {
     int n1m, n2m, n3m;    /* interior cells per axis */
     double L1, L2, L3;    /* domain lengths (x/y spacing L/nm is uniform) */

@@ -1,8 +1,8 @@
 // ppf_fft.hpp -- LDS-resident complex FFT for gfx950 (Stockham autosort).
 //
 // A workgroup transforms several independent lines at once; each line is M
-// complex values in LDS and is served by TPL threads.  Stages use radix 8, 4,
-// 3 or 2, chosen at compile time from M = 2^a or 3*2^a.  Each stage reads its
+// complex values in LDS and is served by TPL threads.  Stages use radix 8, 7,
+// 5, 4, 3 or 2, chosen at compile time from M = 2^a 3^b 5^c 7^d.  Each stage reads its
 // butterfly inputs into registers, barriers, and writes the outputs back in
 // place (Stockham ordering), so one LDS buffer per line suffices and the
 // result lands in natural order.
@@ -31,18 +31,18 @@ struct alignas(PPF_CX_ALIGN ? 2 * sizeof(T) : alignof(T)) cx {
 };
 
 template <typename T>
-__device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+__host__ __device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>
-__device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+__host__ __device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
 template <typename T>
-__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__host__ __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
 // multiply by -i
 template <typename T>
-__device__ __forceinline__ cx<T> cmi(cx<T> a) { return {a.y, -a.x}; }
+__host__ __device__ __forceinline__ cx<T> cmi(cx<T> a) { return {a.y, -a.x}; }
 template <typename T>
-__device__ __forceinline__ cx<T> cconj(cx<T> a) { return {a.x, -a.y}; }
+__host__ __device__ __forceinline__ cx<T> cconj(cx<T> a) { return {a.x, -a.y}; }
 template <typename T>
-__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s}; }
+__host__ __device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s}; }
 
 // ---- swizzled LDS lines ----------------------------------------------------
 // A Stockham stage writes its butterfly outputs R complex values apart
@@ -75,13 +75,18 @@ __device__ __forceinline__ LReal<T> real_view(LLine<T> l) {
 // ---- compile-time radix plan -------------------------------------------
 constexpr bool fft_size_ok(int m) {
     if (m < 1) return false;
-    if (m % 3 == 0) m /= 3;
+    for (int p : {3, 5, 7})
+        while (m % p == 0) m /= p;
     while (m % 2 == 0) m /= 2;
     return m == 1;
 }
-// radix of the next stage given the remaining length
+// radix of the next stage given the remaining length: the odd factors first,
+// then 8s (16 as 4 x 4), 4s and a 2
 constexpr int pick_radix(int rem) {
-    return (rem % 3 == 0) ? 3 : ((rem % 8 == 0 && rem != 16) ? 8 : ((rem % 4 == 0) ? 4 : 2));
+    return (rem % 7 == 0)   ? 7
+           : (rem % 5 == 0) ? 5
+           : (rem % 3 == 0) ? 3
+                            : ((rem % 8 == 0 && rem != 16) ? 8 : ((rem % 4 == 0) ? 4 : 2));
 }
 constexpr int max_radix(int m) {
     int r = 1;
@@ -99,7 +104,7 @@ struct Dft;
 
 template <typename T>
 struct Dft<T, 2> {
-    __device__ __forceinline__ static void run(cx<T> *v) {
+    __host__ __device__ __forceinline__ static void run(cx<T> *v) {
         cx<T> a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
@@ -108,7 +113,7 @@ struct Dft<T, 2> {
 
 template <typename T>
 struct Dft<T, 3> {
-    __device__ __forceinline__ static void run(cx<T> *v) {
+    __host__ __device__ __forceinline__ static void run(cx<T> *v) {
         const T h = T(0.86602540378443864676372317075294);  // sqrt(3)/2
         cx<T> a = v[0], b = v[1], c = v[2];
         cx<T> s = cadd(b, c);
@@ -121,8 +126,59 @@ struct Dft<T, 3> {
 };
 
 template <typename T>
+struct Dft<T, 5> {
+    // X_m = b_m -+ i c_m over the conjugate pairs (1,4), (2,3)
+    __host__ __device__ __forceinline__ static void run(cx<T> *v) {
+        const T c1 = T(0.30901699437494742410229341718281906), c2 = T(-0.80901699437494742410229341718281906);
+        const T s1 = T(0.95105651629515357211643933337938214), s2 = T(0.58778525229247312916870595463907277);
+        const cx<T> a0 = v[0];
+        const cx<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        const cx<T> u1 = csub(v[1], v[4]), u2 = csub(v[2], v[3]);
+        const cx<T> b1 = {a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y};
+        const cx<T> b2 = {a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y};
+        const cx<T> e1 = cmi(cx<T>{s1 * u1.x + s2 * u2.x, s1 * u1.y + s2 * u2.y});  // -i c_1
+        const cx<T> e2 = cmi(cx<T>{s2 * u1.x - s1 * u2.x, s2 * u1.y - s1 * u2.y});  // -i c_2
+        v[0] = cadd(a0, cadd(t1, t2));
+        v[1] = cadd(b1, e1);
+        v[4] = csub(b1, e1);
+        v[2] = cadd(b2, e2);
+        v[3] = csub(b2, e2);
+    }
+};
+
+template <typename T>
+struct Dft<T, 7> {
+    // X_m = b_m -+ i c_m over the conjugate pairs (1,6), (2,5), (3,4)
+    __host__ __device__ __forceinline__ static void run(cx<T> *v) {
+        const T c1 = T(0.62348980185873353052500488400423981), c2 = T(-0.22252093395631440428890256449679476),
+                c3 = T(-0.90096886790241912623610231950744505);
+        const T s1 = T(0.78183148246802980870844452667405775), s2 = T(0.97492791218182360701813168299393122),
+                s3 = T(0.43388373911755812047576833284835875);
+        const cx<T> a0 = v[0];
+        const cx<T> t1 = cadd(v[1], v[6]), t2 = cadd(v[2], v[5]), t3 = cadd(v[3], v[4]);
+        const cx<T> u1 = csub(v[1], v[6]), u2 = csub(v[2], v[5]), u3 = csub(v[3], v[4]);
+        auto lin = [](T a, cx<T> p, T b, cx<T> q, T c, cx<T> r) {
+            return cx<T>{a * p.x + b * q.x + c * r.x, a * p.y + b * q.y + c * r.y};
+        };
+        const cx<T> b1 = cadd(a0, lin(c1, t1, c2, t2, c3, t3));
+        const cx<T> b2 = cadd(a0, lin(c2, t1, c3, t2, c1, t3));
+        const cx<T> b3 = cadd(a0, lin(c3, t1, c1, t2, c2, t3));
+        const cx<T> e1 = cmi(lin(s1, u1, s2, u2, s3, u3));
+        const cx<T> e2 = cmi(lin(s2, u1, -s3, u2, -s1, u3));
+        const cx<T> e3 = cmi(lin(s3, u1, -s1, u2, s2, u3));
+        v[0] = cadd(a0, cadd(t1, cadd(t2, t3)));
+        v[1] = cadd(b1, e1);
+        v[6] = csub(b1, e1);
+        v[2] = cadd(b2, e2);
+        v[5] = csub(b2, e2);
+        v[3] = cadd(b3, e3);
+        v[4] = csub(b3, e3);
+    }
+};
+
+template <typename T>
 struct Dft<T, 4> {
-    __device__ __forceinline__ static void run(cx<T> *v) {
+    __host__ __device__ __forceinline__ static void run(cx<T> *v) {
         cx<T> s0 = cadd(v[0], v[2]), d0 = csub(v[0], v[2]);
         cx<T> s1 = cadd(v[1], v[3]), d1 = cmi(csub(v[1], v[3]));
         v[0] = cadd(s0, s1);
@@ -134,7 +190,7 @@ struct Dft<T, 4> {
 
 template <typename T>
 struct Dft<T, 8> {
-    __device__ __forceinline__ static void run(cx<T> *v) {
+    __host__ __device__ __forceinline__ static void run(cx<T> *v) {
         const T r = T(0.70710678118654752440084436210485);
         cx<T> e[4] = {v[0], v[2], v[4], v[6]};
         cx<T> o[4] = {v[1], v[3], v[5], v[7]};

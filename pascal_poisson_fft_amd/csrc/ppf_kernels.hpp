@@ -954,7 +954,6 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     if (gated_off(z.gate)) return;
     extern __shared__ double zsh[];
     const int n3m = z.n3m;
-    const int NP = P * K;           // padded rows
     constexpr int KS = (K + S - 1) / S;           // table rows per thread
     const int NPAD = KS * (int)blockDim.x;        // >= NP: every thread writes KS entries
     double *am = zsh;               // [NPAD], 0 past n3m

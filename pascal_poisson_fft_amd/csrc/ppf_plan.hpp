@@ -23,8 +23,13 @@ struct AxisTables {
     cx<T> *qk = nullptr;  // exp(-i pi k / 2N), M+1 entries
 };
 
-// Line lengths with compiled transform kernels (x and y extents).
-#define PPF_FOR_SIZES(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+// Line lengths with compiled transform kernels (x and y extents): the even
+// lengths 8..1024 of the form 2^a * {1, 3, 5, 7, 15}, in three groups that
+// compile as separate translation units.
+#define PPF_SIZES_A(X) X(8) X(10) X(12) X(14) X(16) X(20) X(24) X(28) X(30) X(32) X(40) X(48) X(56) X(60) X(64) X(80) X(96)
+#define PPF_SIZES_B(X) X(112) X(120) X(128) X(160) X(192) X(224) X(240) X(256) X(320) X(384)
+#define PPF_SIZES_C(X) X(448) X(480) X(512) X(640) X(768) X(896) X(960) X(1024)
+#define PPF_FOR_SIZES(X) PPF_SIZES_A(X) PPF_SIZES_B(X) PPF_SIZES_C(X)
 
 inline bool size_supported(int n) {
     switch (n) {
@@ -129,5 +134,17 @@ inline int dispatch_y(const ppf_plan *p, T *W, const AxisTables<T> &tb, bool inv
 }
 template <typename T>
 int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hipStream_t s);
+
+// Launchers for one line length N, defined by the size-group translation
+// units (ppf_xpass.inc, ppf_ypass.inc) and called by the dispatchers above.
+template <typename T, int N>
+int xfwd_n(const ppf_plan *p, const double *in, T *W, const AxisTables<T> &tb, bool res, const ZRows &zr,
+           hipStream_t s, int nplanes);
+template <typename T, int N>
+int xinv_n(const ppf_plan *p, const T *W, double *P, const AxisTables<T> &tb, const double *dmx1, bool acc,
+           hipStream_t s, int nplanes);
+template <typename T, int N>
+int y_n(const ppf_plan *p, T *W, const AxisTables<T> &tb, bool inverse, const double *dmx2, hipStream_t s,
+        int nplanes);
 
 }  // namespace ppf

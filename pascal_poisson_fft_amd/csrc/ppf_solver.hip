@@ -159,7 +159,7 @@ const char *ppf_error_string(int code) {
     switch (code) {
         case PPF_OK: return "success";
         case PPF_ERR_INVALID_ARG: return "invalid argument";
-        case PPF_ERR_SIZE: return "unsupported extent (x/y need even 8..1024 of the form 2^a or 3*2^a; z >= 2 even)";
+        case PPF_ERR_SIZE: return "unsupported extent (x/y need even 8..1024 of the form 2^a times 1, 3, 5, 7 or 15; z even, 2..4096)";
         case PPF_ERR_HIP: return "HIP runtime error";
         case PPF_ERR_ALLOC: return "device allocation failed";
         case PPF_ERR_UNSUPPORTED: return "unsupported configuration";

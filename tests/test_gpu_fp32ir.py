@@ -56,7 +56,7 @@ BCS = [(False, False), (True, True), (True, False), (False, True)]
 
 @pytest.mark.parametrize("pbc", BCS, ids=["NN", "PP", "PN", "NP"])
 @pytest.mark.parametrize("n1,n2,n3,uniform_z", [(32, 24, 16, False), (64, 64, 64, False), (96, 48, 40, True),
-                                                 (128, 192, 34, False)])
+                                                 (128, 192, 34, False), (160, 112, 20, False)])
 def test_fp32_ir_reaches_tolerance_and_matches_oracle(pbc, n1, n2, n3, uniform_z):
     ax, ay, az, f = _setup(n1, n2, n3, pbc, uniform_z, 1.5, seed=n1 + 5 * n2 + 11 * n3)
     print(f"fp32_ir {n1}x{n2}x{n3} pbc={pbc}", flush=True)

@@ -46,6 +46,19 @@ def test_solve_matches_oracle(pbc, n1, n2, n3):
 
 
 @pytest.mark.parametrize("pbc", BCS, ids=["NN", "PP", "PN", "NP"])
+@pytest.mark.parametrize("n1,n2,n3", [(160, 320, 8), (320, 160, 6), (10, 14, 12), (30, 60, 10), (640, 20, 4),
+                                      (14, 896, 4), (960, 30, 4), (112, 224, 6), (480, 120, 4), (40, 80, 16)])
+def test_radix5_7_lengths_match_oracle(pbc, n1, n2, n3):
+    """Lengths with factors 5, 7 and 15 (radix-5 and radix-7 Stockham stages),
+    which the reference's cuFFT plans accept: 5*2^a (10..640), 7*2^a (14..896),
+    15*2^a (30..960)."""
+    ax, ay, az, f = _problem(n1, n2, n3, pbc, False, 1.5, n1 * 5 + n2 * 7 + n3)
+    ref = O.solve(f, (ax.L, ay.L, az.L), (pbc[0], pbc[1], False), ax.dmx, ay.dmx, az.dmx, az.dx)
+    got = api.solve_host(f, ax, ay, az)
+    assert _rel_l2(got, ref) <= TOL_FP64, _rel_l2(got, ref)
+
+
+@pytest.mark.parametrize("pbc", BCS, ids=["NN", "PP", "PN", "NP"])
 def test_incompatible_rhs_pin_semantics(pbc):
     """RHS without the compatibility projection: the (0,0)-mode pin decides the
     answer; it must match the oracle exactly as the reference's pin does."""
@@ -128,7 +141,10 @@ def test_tdma_many_bitwise_vs_reference_order(nx, ny, nz):
 
 def test_plan_rejects_bad_sizes():
     with pytest.raises(api.PPFError) as e:
-        api.FFTPoisson(10, 8, 8)  # 10 = 2*5 is not 2^a or 3*2^a
+        api.FFTPoisson(22, 8, 8)  # 22 = 2*11: no compiled line length
+    assert e.value.code == -2
+    with pytest.raises(api.PPFError) as e:
+        api.FFTPoisson(8, 1280, 8)  # beyond 1024
     assert e.value.code == -2
     with pytest.raises(api.PPFError) as e:
         api.FFTPoisson(8, 8, 7)  # odd z
