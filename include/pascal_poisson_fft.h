@@ -262,6 +262,14 @@ int ppf_synth_divergence_rhs(double *f, int n1m, int n2m, int n3m, double L1, do
                              const double *tables, int nmodes, int stride, unsigned long long seed, double noise,
                              void *stream);
 
+/* The reference example's right-hand side on the device: f(i,j,k) =
+ * -cos(x1(i) pi) cos(x2(j) pi) cos(x3(k) pi) 3 pi^2 at the grid nodes, i.e.
+ * cuda_pressure_RHS_kernel (01_Fortran/examples/cuda_pressure.f90:66-90).
+ * x1, x2, x3 (device, nm+2 each) are the node arrays x(0:nm+1); f is
+ * (n1m, n2m, n3m). */
+int ppf_example_rhs(double *f, const double *x1, const double *x2, const double *x3, int n1m, int n2m, int n3m,
+                    void *stream);
+
 #ifdef __cplusplus
 }
 #endif

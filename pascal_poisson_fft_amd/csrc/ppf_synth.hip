@@ -13,6 +13,7 @@
 // construction.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "../../include/pascal_poisson_fft.h"
@@ -82,5 +83,38 @@ extern "C" int ppf_synth_divergence_rhs(double *f, int n1m, int n2m, int n3m, do
     const unsigned grid = (unsigned)((N + 255) / 256);
     hipLaunchKernelGGL(k_synth_div, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream), f, n1m, n2m, n3m,
                        (double)n1m / L1, (double)n2m / L2, dx3, tables, nmodes, stride, (uint64_t)seed, noise);
+    return hipGetLastError() == hipSuccess ? PPF_OK : PPF_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------------
+// The reference example's right-hand side on the device
+// (cuda_pressure_RHS_kernel, 01_Fortran/examples/cuda_pressure.f90:66-90):
+// prhs(i,j,k) = -cos(x1(i) pi) cos(x2(j) pi) cos(x3(k) pi) 3 pi^2 at the grid
+// nodes, in the reference's left-to-right product order.
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void __launch_bounds__(256)
+k_example_rhs(double *__restrict__ f, const double *__restrict__ x1, const double *__restrict__ x2,
+              const double *__restrict__ x3, int n1m, int n2m, int n3m) {
+    const double pi = 3.14159265358979323846264338327950288;
+    const size_t n = (size_t)n1m * n2m * n3m;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % n1m);
+        const size_t r = q / n1m;
+        const int j = (int)(r % n2m), k = (int)(r / n2m);
+        // Fortran element i of x(0:) is x[i]; cells are 1-based
+        const double v = cos(x1[i + 1] * pi) * cos(x2[j + 1] * pi) * cos(x3[k + 1] * pi) * 3.0 * pi * pi;
+        f[q] = -v;
+    }
+}
+}  // namespace
+
+extern "C" int ppf_example_rhs(double *f, const double *x1, const double *x2, const double *x3, int n1m, int n2m,
+                               int n3m, void *stream) {
+    if (!f || !x1 || !x2 || !x3 || n1m <= 0 || n2m <= 0 || n3m <= 0) return PPF_ERR_INVALID_ARG;
+    const size_t n = (size_t)n1m * n2m * n3m;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_example_rhs, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream), f, x1, x2, x3, n1m,
+                       n2m, n3m);
     return hipGetLastError() == hipSuccess ? PPF_OK : PPF_ERR_HIP;
 }

@@ -2,8 +2,10 @@
 !>
 !> Follows the GPU sequence of the reference example (01_Fortran/examples/
 !> main.f90:38-61,109-142,175-190): read the namelist input, build the grid,
-!> allocate device arrays, fill the RHS, create + initialise the plan, time
-!> one cuda_Poisson_FFT_1D, copy P back, write the continue file, clean up.
+!> allocate device arrays, create + initialise the plan, load the continue
+!> file when ContinueFilein is set (main.f90:136-139), then time the RHS
+!> kernel + one cuda_Poisson_FFT_1D + the copy of P back (main.f90:44-61),
+!> write the continue file and the Tecplot field, clean up.
 !> The reference's MPI process grid collapses to one process: the input's
 !> MPI_procs entries are read and ignored (the solution does not depend on
 !> the decomposition).
@@ -12,7 +14,10 @@
 !>
 !> Input: the namelists of 01_Fortran/run/PARA_INPUT_GPU.dat, read by
 !> global_inputpara (global.f90:59-120).  Output: timing and the relative FP64
-!> residual on stdout; with ContinueFileout, dir_cont_fileout//'cont_P.bin'
+!> residual on stdout, and two checks: the device RHS against the same
+!> formula evaluated on the host, and -- with ContinueFilein -- the field read
+!> from dir_cont_filein//'cont_P.bin' against the new solution.  With
+!> ContinueFileout, dir_cont_fileout//'cont_P.bin'
 !> holds the interior P(1:n1m,1:n2m,1:n3m) as raw FP64 in Fortran order, the
 !> layout of cuda_Post_FileOut_Continue_Post_Reassembly_IO (cuda_post.f90:260-292);
 !> dir_instantfield//'P_000.PLT' is the Tecplot ASCII field of
@@ -20,8 +25,18 @@
 !> to the grid nodes, one "X Y Z P" line per node in E11.4.
 program ppf_example
     use iso_fortran_env, only : int64
+    use iso_c_binding
     use fft_poisson
     implicit none
+
+    interface
+        !> The example's RHS kernel on the device (include/pascal_poisson_fft.h)
+        integer(c_int) function c_example_rhs(f, x1, x2, x3, n1m, n2m, n3m, stream) bind(C, name='ppf_example_rhs')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: f, x1, x2, x3, stream
+            integer(c_int), value :: n1m, n2m, n3m
+        end function
+    end interface
 
     real(rp) :: L1, L2, L3, gamma1, gamma2, gamma3
     integer :: n1m, n2m, n3m, np1, np2, np3, uniform1, uniform2, uniform3
@@ -43,11 +58,12 @@ program ppf_example
     character(len=512) :: input
     integer :: u, ios, n1, n2, n3, i, j, k, sweeps
     real(rp), allocatable :: x1(:), x2(:), x3(:), dx1(:), dx2(:), dx3(:), dmx1(:), dmx2(:), dmx3(:)
-    real(rp), allocatable :: prhs(:, :, :), P(:, :, :)
+    real(rp), allocatable :: prhs(:, :, :), rhs_dev(:, :, :), P(:, :, :), Pcont(:, :, :)
     real(rp), pointer :: PRHS_d(:, :, :) => null(), P_d(:, :, :) => null()
     real(rp), pointer :: dmx1_d(:) => null(), dmx2_d(:) => null(), dmx3_d(:) => null(), dx3_d(:) => null()
+    real(rp), pointer :: x1_d(:) => null(), x2_d(:) => null(), x3_d(:) => null()
     integer(int64) :: c0, c1, rate
-    real(rp) :: rel, rel_solver
+    real(rp) :: rel, rel_solver, rhs_diff
 
     ! defaults of the reference input file (01_Fortran/run/PARA_INPUT_GPU.dat)
     L1 = 2.0_rp; L2 = 2.0_rp; L3 = 2.0_rp
@@ -111,9 +127,44 @@ program ppf_example
     call ppf_upload_1d(dmx2_d, dmx2)
     call ppf_upload_1d(dmx3_d, dmx3)
     call ppf_upload_1d(dx3_d, dx3)
+    ! node coordinates for the RHS kernel (cuda_subdomain.f90: x1_d, x2_d, x3_d)
+    call ppf_device_alloc_1d(x1_d, 0, n1)
+    call ppf_device_alloc_1d(x2_d, 0, n2)
+    call ppf_device_alloc_1d(x3_d, 0, n3)
+    call ppf_upload_1d(x1_d, x1)
+    call ppf_upload_1d(x2_d, x2)
+    call ppf_upload_1d(x3_d, x3)
+    allocate(prhs(n1m, n2m, n3m), rhs_dev(n1m, n2m, n3m), P(0:n1, 0:n2, 0:n3))
+    P = 0.0_rp
+    call ppf_upload_3d(P_d, P)
 
-    ! RHS of the reference example at the grid nodes (cuda_pressure.f90:66-90)
-    allocate(prhs(n1m, n2m, n3m), P(0:n1, 0:n2, 0:n3))
+    call fft_poisson_plan_cuda_create(0, 0, 0, 1, 1, 1, -1, -1, -1, -1, -1, -1, 0, 0, 0, &
+                                      n1, n2, n3, n1, n2, n3, L1, L2, L3, pbc1, pbc2, pbc3, &
+                                      dim3(thread_in_x_pascal, thread_in_y_pascal, 1), &
+                                      dim3(thread_in_fft_pascal, thread_in_y_pascal, 1))
+    call cuda_Poisson_FFT_initial()
+    if (ContinueFilein) then
+        ! cuda_Post_FileIn_Continue_Post_Reassembly_IO (cuda_post.f90:224-258) on
+        ! one process: the interior from one direct-access record, periodic
+        ! ghosts wrapped (mpi_subdomain_ghostcell_update_real), then P_d = P
+        call read_continue_file(trim(dir_cont_filein)//'cont_P.bin', P)
+        allocate(Pcont(n1m, n2m, n3m))
+        Pcont = P(1:n1m, 1:n2m, 1:n3m)
+        call ppf_upload_3d(P_d, P)
+        write(*, '(a)') 'Read continue file using post reassembly IO'
+    end if
+    write(*, '(a)') '[[[============== all setup finished ==============]]]'
+
+    ! timed as the reference times it (main.f90:44-61): RHS kernel, solve, D2H
+    call system_clock(c0, rate)
+    call cuda_pressure_RHS(PRHS_d)
+    call cuda_Poisson_FFT_1D(PRHS_d, P_d, dmx1_d, dmx2_d, dmx3_d, dx3_d)
+    call ppf_download_3d(P, P_d)
+    call system_clock(c1)
+    write(*, '(a,es14.6)') ' Total calculation time: ', real(c1 - c0, rp) / real(rate, rp)
+
+    ! the device RHS against the same formula on the host (cuda_pressure.f90:87)
+    call ppf_download_3d(rhs_dev, PRHS_d)
     do k = 1, n3m
         do j = 1, n2m
             do i = 1, n1m
@@ -121,20 +172,10 @@ program ppf_example
             end do
         end do
     end do
-    call ppf_upload_3d(PRHS_d, prhs)
-
-    call fft_poisson_plan_cuda_create(0, 0, 0, 1, 1, 1, -1, -1, -1, -1, -1, -1, 0, 0, 0, &
-                                      n1, n2, n3, n1, n2, n3, L1, L2, L3, pbc1, pbc2, pbc3, &
-                                      dim3(thread_in_x_pascal, thread_in_y_pascal, 1), &
-                                      dim3(thread_in_fft_pascal, thread_in_y_pascal, 1))
-    call cuda_Poisson_FFT_initial()
-    write(*, '(a)') '[[[============== all setup finished ==============]]]'
-
-    call system_clock(c0, rate)
-    call cuda_Poisson_FFT_1D(PRHS_d, P_d, dmx1_d, dmx2_d, dmx3_d, dx3_d)
-    call ppf_download_3d(P, P_d)
-    call system_clock(c1)
-    write(*, '(a,es14.6)') ' Total calculation time: ', real(c1 - c0, rp) / real(rate, rp)
+    rhs_diff = maxval(abs(rhs_dev - prhs)) / maxval(abs(prhs))
+    write(*, '(a,es14.6)') ' RHS device vs host (max rel diff): ', rhs_diff
+    if (ContinueFilein) write(*, '(a,es14.6)') ' Continue-file field vs solution (max abs diff): ', &
+        maxval(abs(Pcont - P(1:n1m, 1:n2m, 1:n3m)))
 
     rel = fft_poisson_residual(PRHS_d, P_d, dmx3_d, dx3_d)
     call fft_poisson_last_stats(sweeps, rel_solver)
@@ -159,9 +200,51 @@ program ppf_example
     call ppf_device_free_1d(dmx2_d)
     call ppf_device_free_1d(dmx3_d)
     call ppf_device_free_1d(dx3_d)
+    call ppf_device_free_1d(x1_d)
+    call ppf_device_free_1d(x2_d)
+    call ppf_device_free_1d(x3_d)
     write(*, '(a)') '[Main] Simulation complete. (GPU)'
 
 contains
+
+    !> The example's RHS on the device (cuda_pressure_RHS,
+    !> cuda_pressure.f90:66-90): launched on the default stream, ordered before
+    !> the solve.
+    subroutine cuda_pressure_RHS(PRHS_d)
+        real(rp), pointer, intent(in) :: PRHS_d(:, :, :)
+        call ppf_check(c_example_rhs(c_loc(PRHS_d(1, 1, 1)), c_loc(x1_d(0)), c_loc(x2_d(0)), c_loc(x3_d(0)), &
+                                     int(n1m, c_int), int(n2m, c_int), int(n3m, c_int), c_null_ptr), 'cuda_pressure_RHS')
+    end subroutine cuda_pressure_RHS
+
+    !> Interior P(1:n1m,1:n2m,1:n3m) from the continue file -- raw FP64 in
+    !> Fortran order, one record -- and the one-process ghost exchange: wrap
+    !> along periodic axes; wall ghosts keep their values.
+    subroutine read_continue_file(path, P)
+        character(len=*), intent(in) :: path
+        real(rp), intent(inout) :: P(0:, 0:, 0:)
+        real(rp), allocatable :: var_all(:, :, :)
+        integer :: u, ios, nx, ny, nz
+        nx = size(P, 1) - 2; ny = size(P, 2) - 2; nz = size(P, 3) - 2
+        allocate(var_all(nx, ny, nz))
+        open(newunit=u, file=path, form='unformatted', access='direct', recl=int(nx, int64) * ny * nz * &
+             (storage_size(var_all) / 8), status='old', action='read', iostat=ios)
+        if (ios /= 0) then
+            write(*, '(a,a)') 'cannot open continue file ', path
+            error stop 1
+        end if
+        read(u, rec=1) var_all
+        close(u)
+        P(1:nx, 1:ny, 1:nz) = var_all
+        if (pbc1) then
+            P(0, :, :) = P(nx, :, :); P(nx + 1, :, :) = P(1, :, :)
+        end if
+        if (pbc2) then
+            P(:, 0, :) = P(:, ny, :); P(:, ny + 1, :) = P(:, 1, :)
+        end if
+        if (pbc3) then
+            P(:, :, 0) = P(:, :, nz); P(:, :, nz + 1) = P(:, :, 1)
+        end if
+    end subroutine read_continue_file
 
     !> Tecplot ASCII output of the reference example (cuda_post.f90:294-331,
     !> calc_C2g :339-353): the value at node (x1(i), x2(j), x3(k)), i = 1..n1m
