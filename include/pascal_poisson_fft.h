@@ -143,13 +143,18 @@ int ppf_solve_async(ppf_plan *plan, const double *prhs, double *p, const double 
  * plan's tol (tol >= 0).  FP64 plans: sweeps 0, rel_residual -1. */
 int ppf_solve_async_result(ppf_plan *plan, void *stream, ppf_stats *stats);
 
-/* Autotuner: times the z-pass kernel shapes (partitioned Thomas with K rows
- * per thread and S columns per workgroup, and the pipelined wavefront cyclic
- * reduction) on this plan's extents with full solves of the given data, keeps
- * the fastest for later solves and returns its id in *zvariant (may be NULL).
- * p is overwritten.  Synchronises the stream.  Replaces the reference's fixed
- * thread shapes threads_tdma / threads_fft (fft_3d_poisson.f90:70-89), which
- * the caller had to choose by hand. */
+/* Autotuner: times, with full solves of the given data on this plan's
+ * extents, the x- and y-pass variants (threads per line: one radix-R
+ * butterfly per thread per stage, or half the threads with two) and the
+ * z-pass kernel shapes (partitioned Thomas with K rows per thread and S
+ * columns per workgroup, pipelined wavefront cyclic reduction); keeps the
+ * fastest of each pass for later solves and returns the z shape's id in
+ * *zvariant (may be NULL).  Results are cached per problem shape (extents,
+ * boundary conditions, precision) for the life of the process: a second plan
+ * of a tuned shape is set without timing.  p is overwritten.  Synchronises
+ * the stream.  Replaces the reference's fixed thread shapes threads_tdma /
+ * threads_fft (fft_3d_poisson.f90:70-89) and its fixed TDMA launch
+ * (pascal_tdma_cuda.f90:168-199), which the caller had to choose by hand. */
 int ppf_plan_tune(ppf_plan *plan, const double *prhs, double *p, const double *dmx1, const double *dmx2,
                   const double *dmx3, const double *dx3, void *stream, int *zvariant);
 
@@ -157,6 +162,12 @@ int ppf_plan_tune(ppf_plan *plan, const double *prhs, double *p, const double *d
  * the default chosen by n3m).  A shape that does not fit the plan's z extent
  * makes later solves return PPF_ERR_UNSUPPORTED. */
 int ppf_plan_set_zvariant(ppf_plan *plan, int zvariant);
+
+/* The plan's pass variants: variants[0] x pass (0/1), [1] y pass (0/1), [2] z
+ * shape (as ppf_plan_set_zvariant). */
+int ppf_plan_variants(const ppf_plan *plan, int variants[3]);
+/* Pins all three (x and y: 0 or 1). */
+int ppf_plan_set_variants(ppf_plan *plan, int xvariant, int yvariant, int zvariant);
 
 /* ---- batched solves of independent problems ----------------------------- */
 /* Device buffers of one problem of a batch (as in ppf_solve). */

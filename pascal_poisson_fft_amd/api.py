@@ -99,6 +99,8 @@ def lib() -> C.CDLL:
     L.ppf_solve_async_result.argtypes = [vp, vp, C.POINTER(Stats)]
     L.ppf_plan_tune.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(I)]
     L.ppf_plan_set_zvariant.argtypes = [vp, I]
+    L.ppf_plan_variants.argtypes = [vp, C.POINTER(I)]
+    L.ppf_plan_set_variants.argtypes = [vp, I, I, I]
     L.ppf_batch_create.argtypes = [I, C.POINTER(Desc), C.POINTER(BatchIO), I, C.POINTER(vp)]
     L.ppf_batch_solve.argtypes = [vp, vp, C.POINTER(Stats)]
     L.ppf_batch_destroy.argtypes = [vp]
@@ -283,6 +285,15 @@ class FFTPoisson:
 
     def set_zvariant(self, v: int) -> None:
         _check(lib().ppf_plan_set_zvariant(self.handle, int(v)), "ppf_plan_set_zvariant")
+
+    def variants(self) -> tuple:
+        """(x-pass variant, y-pass variant, z shape) the plan runs."""
+        v = (C.c_int * 3)()
+        _check(lib().ppf_plan_variants(self.handle, v), "ppf_plan_variants")
+        return tuple(int(x) for x in v)
+
+    def set_variants(self, xv: int, yv: int, zv: int) -> None:
+        _check(lib().ppf_plan_set_variants(self.handle, int(xv), int(yv), int(zv)), "ppf_plan_set_variants")
 
     def clean(self) -> None:
         if self.handle:

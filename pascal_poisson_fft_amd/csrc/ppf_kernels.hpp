@@ -52,11 +52,6 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
     else *p = v;
 }
 
-// Divides the y passes' threads per line (smaller workgroups, more of them
-// per CU).
-#ifndef PPF_YTPL_DIV
-#define PPF_YTPL_DIV 1
-#endif
 
 // Scalars of padding between consecutive LDS lines (odd: lines start on
 // different banks; 2: lines stay 16-byte aligned).
@@ -108,11 +103,14 @@ struct LdsLines {
 // ---------------------------------------------------------------------------
 // pass 1: x-forward.  One LDS line per prhs row; LPB rows per workgroup.
 // PER: periodic x (R2C, Lx = N+2) else DCT-II (Lx = N).
+// XV: the tuner's variant -- 0: one radix-R butterfly per thread per stage;
+// 1: half the threads per line, two butterflies each (twice the lines per
+// workgroup).
 // ---------------------------------------------------------------------------
-template <typename T, int N, bool PER>
+template <typename T, int N, bool PER, int XV = 0>
 struct XGeom {
     using G = LineGeom<N, PER>;
-    static constexpr int TPL = G::TPL;
+    static constexpr int TPL = (G::TPL >> XV) > 0 ? (G::TPL >> XV) : 1;
     static constexpr int LPB = (256 / TPL) > 0 ? (256 / TPL) : 1;
     static constexpr int NT = LPB * TPL;
     using L = LdsLines<T, G::M, G::STRIDE, LPB>;
@@ -202,13 +200,13 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 // RES: the refinement form -- the line is the residual r = f - L x that
 // k_residual stored in FP32 in this line's own W row (in is unused), and the
 // transform overwrites it there.
-template <typename T, int N, bool PER, bool RES = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XGeom<T, N, PER>::WPE)))
+template <typename T, int N, bool PER, bool RES = false, int XV = 0>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XGeom<T, N, PER, XV>::WPE)))
 k_xfwd(const double *__restrict__ in, T *W, int nlines, int n2m, int Ly,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk,
        ZRows zr, const int *__restrict__ gate) {
     if (gated_off(gate)) return;
-    using X = XGeom<T, N, PER>;
+    using X = XGeom<T, N, PER, XV>;
     using G = typename X::G;
     using L = typename X::L;
     constexpr int M = G::M, TPL = X::TPL, LPB = X::LPB, NT = X::NT;
@@ -314,14 +312,14 @@ k_xfwd(const double *__restrict__ in, T *W, int nlines, int n2m, int Ly,
 #ifndef PPF_XIR_CAP_N
 #define PPF_XIR_CAP_N 512
 #endif
-template <typename T, int N, bool PER, bool ACC = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ACC ? (N >= PPF_XIR_CAP_N ? PPF_XACC_WPE : 1) : XGeom<T, N, PER>::WPE)))
+template <typename T, int N, bool PER, bool ACC = false, int XV = 0>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ACC ? (N >= PPF_XIR_CAP_N ? PPF_XACC_WPE : 1) : XGeom<T, N, PER, XV>::WPE)))
 k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk,
        const double *__restrict__ dmx1, const double *__restrict__ mean, const double *__restrict__ xsum,
        double inv_cells, const int *__restrict__ gate) {
     if (gated_off(gate)) return;
-    using X = XGeom<T, N, PER>;
+    using X = XGeom<T, N, PER, XV>;
     using G = typename X::G;
     using L = typename X::L;
     constexpr int M = G::M, TPL = X::TPL, LPB = X::LPB, NT = X::NT;
@@ -442,15 +440,16 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
 // passes 2 and 4: y-forward / y-inverse on a tile of TX scalar columns of one
 // plane.  MODE 0: DCT on real columns; 1: C2C on complex columns (P-P);
 // 2: R2C / C2R on real columns (N-P, spectral rows 0..N+1 as re/im pairs).
+// YV: the tuner's variant -- threads per line halved YV times (each thread
+// then runs 2^YV butterflies per stage; fewer threads per tile, fewer rounds).
 // ---------------------------------------------------------------------------
-template <typename T, int N, int MODE>
+template <typename T, int N, int MODE, int YV = 0>
 struct YGeom {
     static constexpr bool CPLX = MODE == 1;
     static constexpr int M = CPLX ? N : N / 2;
-    // threads per line: one radix-R butterfly each per stage, divided by
-    // PPF_YTPL_DIV (each thread then runs that many butterflies)
+    // threads per line: one radix-R butterfly each per stage (YV = 0)
     static constexpr int TPL0 = (M / max_radix(M)) > 0 ? (M / max_radix(M)) : 1;
-    static constexpr int TPL = TPL0 / PPF_YTPL_DIV > 0 ? TPL0 / PPF_YTPL_DIV : 1;
+    static constexpr int TPL = (TPL0 >> YV) > 0 ? (TPL0 >> YV) : 1;
     static constexpr int S = CPLX ? M + 1 : (MODE == 2 ? M + 1 : M);
     // scalar columns per tile: PPF_YROW_BYTES-wide rows (halved only if the
     // LDS tile -- lines + tables -- would exceed the CU's 160 KiB); lines
@@ -490,12 +489,12 @@ struct YGeom {
     static constexpr int WPE = PPF_YWPE;
 };
 
-template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(YGeom<T, N, MODE>::WPE)))
+template <typename T, int N, int MODE, int YV = 0>
+__global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(YGeom<T, N, MODE, YV>::WPE)))
 k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
        const cx<T> *__restrict__ qk, const int *__restrict__ gate) {
     if (gated_off(gate)) return;
-    using Y = YGeom<T, N, MODE>;
+    using Y = YGeom<T, N, MODE, YV>;
     using L = typename Y::L;
     constexpr int M = Y::M, TPL = Y::TPL, TX = Y::TX, NT = Y::NT;
     constexpr int TP = TX / 2;  // column pairs: global accesses are 2 adjacent scalars
@@ -626,12 +625,12 @@ k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
     }
 }
 
-template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(YGeom<T, N, MODE>::WPE)))
+template <typename T, int N, int MODE, int YV = 0>
+__global__ void __launch_bounds__(kMaxNT) __attribute__((amdgpu_waves_per_eu(YGeom<T, N, MODE, YV>::WPE)))
 k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk,
        const cx<T> *__restrict__ qk, const double *__restrict__ dmx2, const int *__restrict__ gate) {
     if (gated_off(gate)) return;
-    using Y = YGeom<T, N, MODE>;
+    using Y = YGeom<T, N, MODE, YV>;
     using L = typename Y::L;
     constexpr int M = Y::M, TPL = Y::TPL, TX = Y::TX, NT = Y::NT;
     constexpr int NR = MODE == 2 ? N + 2 : N;

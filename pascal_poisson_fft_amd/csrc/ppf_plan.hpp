@@ -79,6 +79,8 @@ struct ppf_plan {
     int mean_kx, mean_jp;
     double mean_scale;
     int zvariant;          // z-pass kernel shape (0 = chosen by n3m)
+    int xvariant = 0;      // x-pass threads per line: 0 one butterfly per stage, 1 two
+    int yvariant = 0;      // y-pass threads per line: 0 one butterfly per stage, 1 two
     // device memory
     double *W = nullptr;   // spectral workspace (FP64 plans)
     float *W32 = nullptr;  // spectral workspace (FP32_IR plans)
@@ -105,6 +107,8 @@ namespace ppf {
 // z-pass shapes: 1-5 and 12-16 partitioned (k_zpart), 6 one thread per line
 // (k_zthomas), 7-11 and 17 pipelined (k_zpipe); 0 = chosen by n3m.
 constexpr int kZVariants = 17;
+// x- and y-pass variants (threads per line, see XGeom / YGeom): 0 and 1.
+constexpr int kXYVariants = 2;
 
 // The z row table of a plan (filled by pass 1 when passed to dispatch_xfwd).
 inline ZRows z_rows(const ppf_plan *p, const double *dx3, const double *dmx3) {

@@ -10,8 +10,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <array>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -23,6 +26,20 @@
 using namespace ppf;
 
 constexpr double kPi = 3.14159265358979323846264338327950288;
+
+namespace {
+// Autotuner results per problem shape, for the life of the process: a second
+// plan of a tuned shape takes them without timing anything.
+struct TuneKey {
+    int n1m, n2m, n3m, pbc1, pbc2, pbc3, precision;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(n1m, n2m, n3m, pbc1, pbc2, pbc3, precision) <
+               std::tie(o.n1m, o.n2m, o.n3m, o.pbc1, o.pbc2, o.pbc3, o.precision);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::array<int, 3>> g_tuned;
+}  // namespace
 
 template <typename T>
 static int dev_alloc(ppf_plan *p, T **ptr, size_t count) {
@@ -473,26 +490,90 @@ int ppf_solve_async_result(ppf_plan *p, void *stream, ppf_stats *stats) {
 int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1, const double *dmx2,
                   const double *dmx3, const double *dx3, void *stream, int *zvariant) {
     if (!p || !prhs || !P || !dmx1 || !dmx2 || !dmx3 || !dx3) return PPF_ERR_INVALID_ARG;
-    // candidates: partitioned (K,S) shapes and the pipelined wavefront-PCR form
+    const TuneKey key{p->d.n1m, p->d.n2m, p->d.n3m, p->d.pbc1 != 0, p->d.pbc2 != 0, p->d.pbc3 != 0, p->d.precision};
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            p->xvariant = it->second[0];
+            p->yvariant = it->second[1];
+            const int rc = ppf_plan_set_zvariant(p, it->second[2]);
+            if (zvariant) *zvariant = p->zvariant;
+            return rc;
+        }
+    }
+    const int keep[3] = {p->xvariant, p->yvariant, p->zvariant};
+    // mean stage times of 2 timed solves after 1 untimed; UNSUPPORTED if the
+    // current shapes do not fit the extents
+    auto time_stages = [&](float out[5]) -> int {
+        for (int i = 0; i < 5; ++i) out[i] = 0.f;
+        for (int rep = 0; rep < 3; ++rep) {
+            float st[5];
+            const int rc = ppf_solve_timed(p, prhs, P, dmx1, dmx2, dmx3, dx3, stream, st);
+            if (rc) return rc;
+            if (rep > 0)
+                for (int i = 0; i < 5; ++i) out[i] += 0.5f * st[i];
+        }
+        return PPF_OK;
+    };
+    auto restore = [&](int rc) {
+        p->xvariant = keep[0];
+        p->yvariant = keep[1];
+        p->zvariant = keep[2];
+        return rc;
+    };
+    // x and y passes: one run per variant pair times both passes (independent
+    // kernels); each pass keeps its faster variant
+    float bx = 0.f, by = 0.f;
+    int vx = 0, vy = 0;
+    p->zvariant = keep[2];
+    for (int v = 0; v < kXYVariants; ++v) {
+        p->xvariant = v;
+        p->yvariant = v;
+        float st[5];
+        const int rc = time_stages(st);
+        if (rc) return restore(rc);
+        if (v == 0 || st[0] + st[4] < bx) { bx = st[0] + st[4]; vx = v; }
+        if (v == 0 || st[1] + st[3] < by) { by = st[1] + st[3]; vy = v; }
+    }
+    p->xvariant = vx;
+    p->yvariant = vy;
+    // z pass: the partitioned (K,S) shapes and the pipelined wavefront forms
     static const int cand[] = {3, 1, 2, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17};
-    const int keep = p->zvariant;
     int best = 0;
     float best_ms = 0.f;
     for (int v : cand) {
         p->zvariant = v;
         float st[5];
-        float z = 0.f;
-        bool ok = true;
-        for (int rep = 0; rep < 3 && ok; ++rep) {
-            const int rc = ppf_solve_timed(p, prhs, P, dmx1, dmx2, dmx3, dx3, stream, st);
-            if (rc == PPF_ERR_UNSUPPORTED) ok = false;
-            else if (rc) { p->zvariant = keep; return rc; }
-            else if (rep > 0) z += st[2];
-        }
-        if (ok && (best == 0 || z < best_ms)) { best = v; best_ms = z; }
+        const int rc = time_stages(st);
+        if (rc == PPF_ERR_UNSUPPORTED) continue;
+        if (rc) return restore(rc);
+        if (best == 0 || st[2] < best_ms) { best = v; best_ms = st[2]; }
     }
-    p->zvariant = best ? best : keep;
+    p->zvariant = best ? best : keep[2];
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tuned[key] = {p->xvariant, p->yvariant, p->zvariant};
+    }
     if (zvariant) *zvariant = p->zvariant;
+    return PPF_OK;
+}
+
+int ppf_plan_variants(const ppf_plan *p, int variants[3]) {
+    if (!p || !variants) return PPF_ERR_INVALID_ARG;
+    variants[0] = p->xvariant;
+    variants[1] = p->yvariant;
+    variants[2] = p->zvariant;
+    return PPF_OK;
+}
+
+int ppf_plan_set_variants(ppf_plan *p, int xvariant, int yvariant, int zvariant) {
+    if (!p || xvariant < 0 || xvariant >= kXYVariants || yvariant < 0 || yvariant >= kXYVariants)
+        return PPF_ERR_INVALID_ARG;
+    const int rc = ppf_plan_set_zvariant(p, zvariant);
+    if (rc) return rc;
+    p->xvariant = xvariant;
+    p->yvariant = yvariant;
     return PPF_OK;
 }
 

@@ -105,3 +105,53 @@ def test_long_z_lines_use_one_thread_per_line(pbc):
     res = O.rel_residual(f, got, L, (pbc[0], pbc[1]), az.dmx, az.dx)
     bar = _res_bar(f, ref, L, pbc, az)
     assert err <= _tol(n3) and res <= bar, (err, res, bar)
+
+
+@pytest.mark.parametrize("pbc", [(False, False), (True, True), (True, False), (False, True)],
+                         ids=["NN", "PP", "PN", "NP"])
+@pytest.mark.parametrize("n1,n2,n3", [(512, 64, 8), (64, 512, 8), (160, 96, 12), (24, 1024, 4), (1024, 16, 4)])
+def test_xy_variants_match_oracle(pbc, n1, n2, n3):
+    """Every x- and y-pass variant the tuner can pick (threads per line: one
+    butterfly per stage, or half the threads with two) against the oracle,
+    at 1e-12, for every boundary branch."""
+    L, ax, ay, az, f, ref = _case(n1, n2, n3, pbc, seed=n1 + n2 + n3)
+    plan = api.FFTPoisson(n1, n2, n3, L, (pbc[0], pbc[1], False))
+    dev = api.Problem3D(f, ax, ay, az)
+    try:
+        for xv, yv in ((0, 1), (1, 0), (1, 1)):
+            print(f"{n1}x{n2}x{n3} pbc={pbc} xv={xv} yv={yv}", flush=True)
+            plan.set_variants(xv, yv, 0)
+            assert plan.variants() == (xv, yv, 0)
+            dev.p.fill_zero()
+            plan.solve(dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3)
+            api.sync()
+            err = float(np.linalg.norm(dev.p.download() - ref) / np.linalg.norm(ref))
+            assert err <= 1e-12, (xv, yv, err)
+    finally:
+        dev.free()
+        plan.clean()
+
+
+def test_tuner_caches_per_shape_and_keeps_parity():
+    """The tuner picks one variant per pass; a second plan of the same shape
+    gets the same picks from the cache; the tuned solve matches the oracle."""
+    n1, n2, n3, pbc = 96, 64, 40, (True, False)
+    L, ax, ay, az, f, ref = _case(n1, n2, n3, pbc, seed=5)
+    dev = api.Problem3D(f, ax, ay, az)
+    plans = [api.FFTPoisson(n1, n2, n3, L, (pbc[0], pbc[1], False)) for _ in range(2)]
+    try:
+        args = (dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3)
+        z = plans[0].tune(*args)
+        v = plans[0].variants()
+        print(f"tuned variants {v}", flush=True)
+        assert v[2] == z and v[0] in (0, 1) and v[1] in (0, 1) and z > 0
+        assert plans[1].tune(*args) == z and plans[1].variants() == v
+        for pl in plans:
+            dev.p.fill_zero()
+            pl.solve(*args)
+            api.sync()
+            assert float(np.linalg.norm(dev.p.download() - ref) / np.linalg.norm(ref)) <= 1e-12
+    finally:
+        for pl in plans:
+            pl.clean()
+        dev.free()
