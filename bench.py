@@ -171,9 +171,11 @@ def batch_leg(api, W, reps: int = 5):
         out["fp32_ir_batch"] = {"ms_per_batch": round(tb * 1e3, 3), "solves_per_s": round(len(probs) / tb, 1),
                                 "mpts_per_s": round(npts / tb / 1e6, 1), "max_sweeps": max(s for s, _ in st),
                                 "max_rel_residual": max(res)}
-        # the same batch in plain FP64 (graph replay, no refinement)
+        # the same batch in plain FP64 (one graph branch per problem, no refinement)
         spec64 = [dict(sp, precision=api.PPF_PREC_FP64) for sp in spec]
+        t0 = time.perf_counter()
         b = api.Batch(list(zip(spec64, devs)))
+        create_s = time.perf_counter() - t0
         b.solve()
         b.solve()
         api.sync()
@@ -186,7 +188,8 @@ def batch_leg(api, W, reps: int = 5):
         b.destroy()
         out["fp64_batch"] = {"ms_per_batch": round(t64 * 1e3, 3), "solves_per_s": round(len(probs) / t64, 1),
                              "mpts_per_s": round(npts / t64 / 1e6, 1),
-                             "max_rel_residual": max(r for _, r in st64)}
+                             "roofline_frac": round(BYTES_PER_PT_SOLVE * npts / t64 / (HBM_PEAK_GBS * 1e9), 4),
+                             "create_s": round(create_s, 2), "max_rel_residual": max(r for _, r in st64)}
         plans = [api.FFTPoisson(p.rhs.shape[2], p.rhs.shape[1], p.rhs.shape[0], p.L, p.pbc) for p in probs]
         for pl, d in zip(plans, devs):
             pl.solve(d.rhs, d.p, d.dmx1, d.dmx2, d.dmx3, d.dx3)

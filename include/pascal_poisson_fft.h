@@ -178,7 +178,7 @@ typedef struct ppf_batch_io {
 } ppf_batch_io;
 
 #define PPF_BATCH_NO_GRAPH 1 /* launch every call directly (no hipGraph) */
-#define PPF_BATCH_NO_TUNE 2  /* skip the per-shape z-pass autotuner */
+#define PPF_BATCH_NO_TUNE 2  /* skip the per-shape autotuner */
 
 typedef struct ppf_batch ppf_batch;
 

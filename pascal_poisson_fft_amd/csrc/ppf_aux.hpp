@@ -1,6 +1,7 @@
 // ppf_aux.hpp -- small non-template kernels of the solve path (z row table,
 // reductions, the device-side refinement test, ghost faces, the copy
-// yardstick).  Included by one translation unit only.
+// yardstick), with internal linkage so several translation units may include
+// them.
 #pragma once
 
 #include "ppf_kernels.hpp"
@@ -10,7 +11,7 @@ namespace ppf {
 // z row coefficients for one solve as a table (read by the residual forms and
 // the pipelined z kernel): zam[k], zap[k] (k < n3m), then zap[n3m..n3m+3] =
 // the wall extrapolation coefficients.
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_zcoef(const double *__restrict__ dx3, const double *__restrict__ dmx3, int n3m, double *__restrict__ zam,
         double *__restrict__ zap) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -20,7 +21,7 @@ k_zcoef(const double *__restrict__ dx3, const double *__restrict__ dmx3, int n3m
 
 // Totals of the per-workgroup triples (sum r^2, sum f^2, sum x) in a fixed
 // order (deterministic).
-__global__ void __launch_bounds__(256) k_reduce3(const double *__restrict__ partial, int n, double *__restrict__ out,
+static __global__ void __launch_bounds__(256) k_reduce3(const double *__restrict__ partial, int n, double *__restrict__ out,
                                                  const int *__restrict__ gate) {
     if (gated_off(gate)) return;
     __shared__ double red[256];
@@ -47,11 +48,11 @@ __global__ void __launch_bounds__(256) k_reduce3(const double *__restrict__ part
 // (count = 1 after a gated sweep) and closes the gate once the residual is
 // <= tol (a NaN residual keeps it open, so the sweeps run out and the caller
 // sees the miss).  One thread each.
-__global__ void k_gate_init(int *__restrict__ gate, int sweeps) {
+static __global__ void k_gate_init(int *__restrict__ gate, int sweeps) {
     gate[0] = 1;
     gate[1] = sweeps;
 }
-__global__ void k_gate_check(const double *__restrict__ red, double tol, int count, int *__restrict__ gate,
+static __global__ void k_gate_check(const double *__restrict__ red, double tol, int count, int *__restrict__ gate,
                              double *__restrict__ res_out) {
     if (gate[0] == 0) return;
     const double rel = red[1] > 0 ? sqrt(red[0] / red[1]) : sqrt(red[0]);
@@ -66,7 +67,7 @@ __global__ void k_gate_check(const double *__restrict__ red, double tol, int cou
 // (cuda_ghostcell_update on one rank, :666-796), over the full (ghosted)
 // ranges of the other two axes.  The reference's order -- walls along x, y,
 // z, then wraps along x, y, z -- is the caller's launch order.
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_ghost_face(double *__restrict__ P, int n1m, int n2m, int n3m, int dir, int wrap, const double *__restrict__ dmx) {
     const size_t L1 = (size_t)n1m + 2, L2 = (size_t)n2m + 2, L3 = (size_t)n3m + 2;
     const size_t nb = dir == 0 ? L2 : L1, nc = dir == 2 ? L2 : L3;
@@ -95,7 +96,7 @@ k_ghost_face(double *__restrict__ P, int n1m, int n2m, int n3m, int dir, int wra
 
 // Plain streaming copy (16-B lanes, grid-stride): the measured-bandwidth
 // yardstick for the pass kernels and the calibration of HBM byte counters.
-__global__ void __launch_bounds__(256) k_copy(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n) {
+static __global__ void __launch_bounds__(256) k_copy(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }

@@ -213,15 +213,17 @@ struct Dft<T, 8> {
 
 // ---- one Stockham stage ---------------------------------------------------
 // line: this thread's LDS line (M complex, padded), t: thread within the line.
+// act = false: a thread with no line in this workgroup; it only takes the
+// barriers (every thread of the workgroup must).
 template <typename T, int M, int TPL, int R, int LS>
-__device__ __forceinline__ void fft_stage(LLine<T> line, int t, const cx<T> *__restrict__ tw) {
+__device__ __forceinline__ void fft_stage(LLine<T> line, int t, const cx<T> *__restrict__ tw, bool act = true) {
     constexpr int S = M / R;                  // butterflies per line
     constexpr int NB = (S + TPL - 1) / TPL;   // butterflies per thread
     cx<T> v[NB][R];
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int b = t + q * TPL;
-        if ((S % TPL == 0) || b < S) {
+        if (act && ((S % TPL == 0) || b < S)) {
 #pragma unroll
             for (int r = 0; r < R; ++r) v[q][r] = line[b + r * S];
         }
@@ -230,7 +232,7 @@ __device__ __forceinline__ void fft_stage(LLine<T> line, int t, const cx<T> *__r
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int b = t + q * TPL;
-        if ((S % TPL == 0) || b < S) {
+        if (act && ((S % TPL == 0) || b < S)) {
             const int j = b % LS;
             const int kk = b / LS;
             if (LS > 1) {
@@ -265,23 +267,23 @@ __device__ __forceinline__ void fft_stage(LLine<T> line, int t, const cx<T> *__r
 
 template <typename T, int M, int TPL, int LS>
 struct FftStages {
-    __device__ __forceinline__ static void run(LLine<T> line, int t, const cx<T> *__restrict__ tw) {
+    __device__ __forceinline__ static void run(LLine<T> line, int t, const cx<T> *__restrict__ tw, bool act) {
         constexpr int R = pick_radix(M / LS);
-        fft_stage<T, M, TPL, R, LS>(line, t, tw);
-        FftStages<T, M, TPL, LS * R>::run(line, t, tw);
+        fft_stage<T, M, TPL, R, LS>(line, t, tw, act);
+        FftStages<T, M, TPL, LS * R>::run(line, t, tw, act);
     }
 };
 template <typename T, int M, int TPL>
 struct FftStages<T, M, TPL, M> {
-    __device__ __forceinline__ static void run(LLine<T>, int, const cx<T> *__restrict__) {}
+    __device__ __forceinline__ static void run(LLine<T>, int, const cx<T> *__restrict__, bool) {}
 };
 
 // Forward FFT (sign -1) of one LDS line, in place, natural order in and out.
 // Every thread of the workgroup must call it (it contains barriers).
 template <typename T, int M, int TPL>
-__device__ __forceinline__ void fft_forward(LLine<T> line, int t, const cx<T> *__restrict__ tw) {
-    static_assert(fft_size_ok(M), "FFT length must be 2^a or 3*2^a");
-    FftStages<T, M, TPL, 1>::run(line, t, tw);
+__device__ __forceinline__ void fft_forward(LLine<T> line, int t, const cx<T> *__restrict__ tw, bool act = true) {
+    static_assert(fft_size_ok(M), "FFT length must be 2^a 3^b 5^c 7^d");
+    FftStages<T, M, TPL, 1>::run(line, t, tw, act);
 }
 
 }  // namespace ppf

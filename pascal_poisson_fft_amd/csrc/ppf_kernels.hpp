@@ -442,8 +442,9 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
 // 2: R2C / C2R on real columns (N-P, spectral rows 0..N+1 as re/im pairs).
 // YV: the tuner's variant -- threads per line halved YV times (each thread
 // then runs 2^YV butterflies per stage; fewer threads per tile, fewer rounds).
+// NTCAP: the most threads a tile may use (more lines go to further rounds).
 // ---------------------------------------------------------------------------
-template <typename T, int N, int MODE, int YV = 0>
+template <typename T, int N, int MODE, int YV = 0, int NTCAP = kMaxNT>
 struct YGeom {
     static constexpr bool CPLX = MODE == 1;
     static constexpr int M = CPLX ? N : N / 2;
@@ -462,11 +463,11 @@ struct YGeom {
     static constexpr int TX = tx_fit(TX0);
     static constexpr int NL = CPLX ? TX / 2 : TX;  // LDS lines per tile
     static constexpr int rounds_for(int r) {
-        return (NL / r) * TPL <= kMaxNT || NL % (2 * r) != 0 ? r : rounds_for(2 * r);
+        return (NL / r) * TPL <= NTCAP || NL % (2 * r) != 0 ? r : rounds_for(2 * r);
     }
     static constexpr int ROUNDS = rounds_for(PPF_YROUNDS);
     static constexpr int NT = NL / ROUNDS * TPL;
-    static_assert(NT <= kMaxNT, "y-pass workgroup exceeds kMaxNT threads");
+    static_assert(NT <= NTCAP, "y-pass workgroup exceeds its thread cap");
     static_assert(lds_bytes(TX) <= 160 * 1024, "y-pass tile exceeds the CU's LDS");
     using L = LdsLines<T, M, S, NL>;
     // Row-blocked mapping (real columns, DCT or R2C): thread (g, c2) owns the

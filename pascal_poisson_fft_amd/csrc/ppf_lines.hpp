@@ -56,7 +56,7 @@ __device__ __forceinline__ int makhoul_pos(int n) {
 // Pair (k, M-k) is handled by one thread.  Result written back as V[0..M]
 // (M+1 complex).  Barrier inside.
 template <typename T, int N, int TPL>
-__device__ __forceinline__ void r2c_post(LLine<T> line, int t, const cx<T> *__restrict__ wk) {
+__device__ __forceinline__ void r2c_post(LLine<T> line, int t, const cx<T> *__restrict__ wk, bool act = true) {
     constexpr int M = N / 2;
     constexpr int NP = M / 2 + 1;  // pairs k = 0..M/2
     constexpr int NB = (NP + TPL - 1) / TPL;
@@ -64,7 +64,7 @@ __device__ __forceinline__ void r2c_post(LLine<T> line, int t, const cx<T> *__re
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int k = t + q * TPL;
-        if (k < NP) {
+        if (act && k < NP) {
             va[q] = line[k];
             vb[q] = line[(M - k) % M];
         }
@@ -73,7 +73,7 @@ __device__ __forceinline__ void r2c_post(LLine<T> line, int t, const cx<T> *__re
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int k = t + q * TPL;
-        if (k < NP) {
+        if (act && k < NP) {
             const cx<T> zk = va[q], zc = cconj(vb[q]);       // Z_k, conj(Z_{M-k})
             const cx<T> E = cscale(cadd(zk, zc), T(0.5));
             const cx<T> O = cmi(cscale(csub(zk, zc), T(0.5)));  // (Z_k - conj Z_{M-k}) / 2i
@@ -97,7 +97,8 @@ __device__ __forceinline__ void r2c_post(LLine<T> line, int t, const cx<T> *__re
 // DCT-II post: from Z (FFT of packed Makhoul sequence) to X[0..N-1] written as
 // the real view of the line.  Barrier inside.
 template <typename T, int N, int TPL>
-__device__ __forceinline__ void dct_post(LLine<T> line, int t, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk) {
+__device__ __forceinline__ void dct_post(LLine<T> line, int t, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk,
+                                         bool act = true) {
     constexpr int M = N / 2;
     constexpr int NP = M / 2 + 1;
     constexpr int NB = (NP + TPL - 1) / TPL;
@@ -105,7 +106,7 @@ __device__ __forceinline__ void dct_post(LLine<T> line, int t, const cx<T> *__re
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int k = t + q * TPL;
-        if (k < NP) {
+        if (act && k < NP) {
             va[q] = line[k];
             vb[q] = line[(M - k) % M];
         }
@@ -115,7 +116,7 @@ __device__ __forceinline__ void dct_post(LLine<T> line, int t, const cx<T> *__re
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int k = t + q * TPL;
-        if (k < NP) {
+        if (act && k < NP) {
             const cx<T> zk = va[q], zc = cconj(vb[q]);
             const cx<T> E = cscale(cadd(zk, zc), T(0.5));
             const cx<T> O = cmi(cscale(csub(zk, zc), T(0.5)));
@@ -147,7 +148,7 @@ __device__ __forceinline__ void dct_post(LLine<T> line, int t, const cx<T> *__re
 // so that a forward FFT followed by conjugation yields the unnormalised
 // inverse z (z_m = v_{2m} + i v_{2m+1}).  Im V_0 and Im V_M are ignored.
 template <typename T, int N, int TPL>
-__device__ __forceinline__ void c2r_pre(LLine<T> line, int t, const cx<T> *__restrict__ wk) {
+__device__ __forceinline__ void c2r_pre(LLine<T> line, int t, const cx<T> *__restrict__ wk, bool act = true) {
     constexpr int M = N / 2;
     constexpr int NP = M / 2 + 1;
     constexpr int NB = (NP + TPL - 1) / TPL;
@@ -155,7 +156,7 @@ __device__ __forceinline__ void c2r_pre(LLine<T> line, int t, const cx<T> *__res
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int k = t + q * TPL;
-        if (k < NP) {
+        if (act && k < NP) {
             va[q] = line[k];
             vb[q] = line[M - k];
         }
@@ -164,7 +165,7 @@ __device__ __forceinline__ void c2r_pre(LLine<T> line, int t, const cx<T> *__res
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int k = t + q * TPL;
-        if (k < NP) {
+        if (act && k < NP) {
             cx<T> Vk = va[q], Vm = vb[q];
             if (k == 0) { Vk.y = T(0); Vm.y = T(0); }
             // Y_k
@@ -191,7 +192,8 @@ __device__ __forceinline__ void c2r_pre(LLine<T> line, int t, const cx<T> *__res
 // DCT inverse pre: from X[0..N-1] (real view of the line) to conj(Y) with
 // V_k = exp(i pi k/2N)(X_k - i X_{N-k}), X_N = 0.  Barrier inside.
 template <typename T, int N, int TPL>
-__device__ __forceinline__ void idct_pre(LLine<T> line, int t, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk) {
+__device__ __forceinline__ void idct_pre(LLine<T> line, int t, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk,
+                                         bool act = true) {
     constexpr int M = N / 2;
     constexpr int NP = M / 2 + 1;
     constexpr int NB = (NP + TPL - 1) / TPL;
@@ -200,7 +202,7 @@ __device__ __forceinline__ void idct_pre(LLine<T> line, int t, const cx<T> *__re
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int k = t + q * TPL;
-        if (k < NP) {
+        if (act && k < NP) {
             x0[q] = re[k];
             x1[q] = (k == 0) ? T(0) : re[N - k];
             x2[q] = re[M - k];
@@ -211,7 +213,7 @@ __device__ __forceinline__ void idct_pre(LLine<T> line, int t, const cx<T> *__re
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int k = t + q * TPL;
-        if (k < NP) {
+        if (act && k < NP) {
             const int k2 = M - k;
             // V_k = conj(q_k) (X_k - i X_{N-k});  V_{M-k} likewise with X_{M-k}, X_{M+k}
             const cx<T> Vk = cmul(cconj(qk[k]), cx<T>{x0[q], -x1[q]});

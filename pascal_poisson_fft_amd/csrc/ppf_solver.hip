@@ -25,7 +25,6 @@
 
 using namespace ppf;
 
-constexpr double kPi = 3.14159265358979323846264338327950288;
 
 namespace {
 // Autotuner results per problem shape, for the life of the process: a second
@@ -53,24 +52,9 @@ static int dev_alloc(ppf_plan *p, T **ptr, size_t count) {
 
 template <typename T>
 static int upload_tables(ppf_plan *p, AxisTables<T> &tb, int N, int kind) {
-    const int M = (kind == AX_C2C) ? N : N / 2;
-    std::vector<cx<T>> tw(M, cx<T>{T(1), T(0)}), wk(M + 1), qk(M + 1);
-    // per-stage FFT twiddles: stage of span LS, radix R at LS-1 + (r-1) LS + j
-    for (int LS = 1; LS < M;) {
-        const int R = pick_radix(M / LS);
-        for (int r = 1; r < R; ++r)
-            for (int j = 0; j < LS; ++j) {
-                long double a = -2.0L * (long double)kPi * (long double)(j * r) / (long double)(LS * R);
-                tw[(size_t)(LS - 1) + (size_t)(r - 1) * LS + j] = {(T)cosl(a), (T)sinl(a)};
-            }
-        LS *= R;
-    }
-    for (int k = 0; k <= M; ++k) {
-        long double a = -2.0L * (long double)kPi * (long double)k / (long double)N;
-        wk[k] = {(T)cosl(a), (T)sinl(a)};
-        long double b = -(long double)kPi * (long double)k / (2.0L * (long double)N);
-        qk[k] = {(T)cosl(b), (T)sinl(b)};
-    }
+    std::vector<cx<T>> tw, wk, qk;
+    host_axis_tables<T>(N, kind, tw, wk, qk);
+    const int M = (int)tw.size();
     int rc;
     if ((rc = dev_alloc(p, &tb.tw, M)) || (rc = dev_alloc(p, &tb.wk, M + 1)) || (rc = dev_alloc(p, &tb.qk, M + 1)))
         return rc;
@@ -78,44 +62,6 @@ static int upload_tables(ppf_plan *p, AxisTables<T> &tb, int N, int kind) {
     PPF_HIP(hipMemcpy(tb.wk, wk.data(), sizeof(cx<T>) * (M + 1), hipMemcpyHostToDevice));
     PPF_HIP(hipMemcpy(tb.qk, qk.data(), sizeof(cx<T>) * (M + 1), hipMemcpyHostToDevice));
     return PPF_OK;
-}
-
-// modified wavenumbers, fft_3d_poisson.f90:288-351 (host for plans, device
-// for the allocation-free solve: one formula)
-__host__ __device__ static inline double eig_neumann(int m, int n, double h) {
-    const double s = sin(0.5 * (double)m * kPi / (double)n);
-    return 4.0 * s * s / (h * h);
-}
-__host__ __device__ static inline double eig_periodic(int m, int n, double h) {
-    return 2.0 * (1.0 - cos(2.0 * (double)m * kPi / (double)n)) / (h * h);
-}
-
-// Eigenvalue and zero-mode flag of workspace scalar column kx (x) and row jp
-// (y).  Periodic x stores the half spectrum as re/im pairs (column kx is mode
-// kx/2); y rows follow the y pass: DCT rows 1..n2m are modes 0..n2m-1, C2C
-// rows 1..n2m are the wrapped modes of the complex FFT, R2C rows 0..n2m+1 are
-// re/im pairs of modes 0..n2m/2.  Unused ghost rows get 0.
-struct EigGeom {
-    int px, ymode, n1m, n2m;
-    double h1, h2;
-};
-__host__ __device__ static inline void eig_col(const EigGeom &g, int kx, double &lam, unsigned char &zero) {
-    const int m = g.px ? kx / 2 : kx;
-    lam = g.px ? eig_periodic(m, g.n1m, g.h1) : eig_neumann(m, g.n1m, g.h1);
-    zero = (m == 0);
-}
-__host__ __device__ static inline void eig_row(const EigGeom &g, int jp, double &lam, unsigned char &zero) {
-    int m;
-    lam = 0.0;
-    zero = 0;
-    if (g.ymode == 0) m = jp - 1;
-    else if (g.ymode == 1) { const int j = jp - 1; m = (j <= g.n2m / 2) ? j : g.n2m - j; }
-    else m = jp / 2;
-    if (m < 0 || m > g.n2m) return;
-    if (g.ymode == 0 && jp > g.n2m) return;
-    if (g.ymode == 1 && (jp < 1 || jp > g.n2m)) return;
-    lam = (g.ymode == 0) ? eig_neumann(m, g.n2m, g.h2) : eig_periodic(m, g.n2m, g.h2);
-    zero = (m == 0);
 }
 
 // Checks of a problem description shared by plans and the allocation-free solve.
@@ -127,32 +73,6 @@ static int desc_check(const ppf_desc &d) {
     if (!size_supported(d.n1m) || !size_supported(d.n2m) || d.n3m < 2 || (d.n3m & 1) || d.n3m > 4096)
         return PPF_ERR_SIZE;
     return PPF_OK;
-}
-
-// Workspace geometry, transform kinds and mean bookkeeping of a problem.
-static void plan_geometry(const ppf_desc &d, ppf_plan *p) {
-    p->d = d;
-    const bool px = d.pbc1 != 0, py = d.pbc2 != 0;
-    p->xkind = px ? AX_R2C : AX_DCT;
-    if (!py) { p->ykind = AX_DCT; p->ymode = 0; }
-    else if (px) { p->ykind = AX_C2C; p->ymode = 1; }
-    else { p->ykind = AX_R2C; p->ymode = 2; }
-    p->Lx = px ? d.n1m + 2 : d.n1m;
-    p->Ly = d.n2m + 2;
-    p->Lz = d.n3m + 2;
-    if (p->ymode == 2) { p->zjlo = 0; p->znjr = d.n2m + 2; }
-    else { p->zjlo = 1; p->znjr = d.n2m; }
-    p->mean_kx = 0;
-    p->mean_jp = (p->ymode == 2) ? 0 : 1;
-    p->zvariant = 0;
-    const double fx = px ? 1.0 : 2.0, fy = (p->ymode == 0) ? 2.0 : 1.0;
-    p->mean_scale = 1.0 / (fx * fy * (double)d.n1m * (double)d.n2m * (double)d.n3m);
-    const bool fp32 = d.precision == PPF_PREC_FP32_IR;
-    p->Lxs = w_stride(p->Lx, fp32 ? (int)sizeof(float) : (int)sizeof(double));
-}
-
-static EigGeom eig_geom(const ppf_plan *p) {
-    return EigGeom{p->d.pbc1 != 0, p->ymode, p->d.n1m, p->d.n2m, p->d.L1 / p->d.n1m, p->d.L2 / p->d.n2m};
 }
 
 static void free_plan(ppf_plan *p) {

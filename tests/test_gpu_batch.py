@@ -82,6 +82,55 @@ def test_batch_eager_then_graph_replays(flags):
             dev.free()
 
 
+def _c4_batch_parity(nprob, seed, flags=0):
+    """nprob FP64 problems drawn from the 96-problem workload's distribution
+    (nx, ny in {32..256}, nz in [32, 384], half periodic x,y, half Neumann,
+    uniform or tanh z) solved as one batch: each problem matches the oracle at
+    1e-12 with a certified residual, and two graph replays reproduce the eager
+    solve bitwise."""
+    probs = W.config4(nprob=nprob, seed=seed)
+    devs = [api.Problem3D(p.rhs, *p.ax) for p in probs]
+    spec = [dict(n1m=p.rhs.shape[2], n2m=p.rhs.shape[1], n3m=p.rhs.shape[0], L=p.L, pbc=p.pbc) for p in probs]
+    refs = [O.solve(p.rhs, p.L, p.pbc, p.ax[0].dmx, p.ax[1].dmx, p.ax[2].dmx, p.ax[2].dx) for p in probs]
+    try:
+        print(f"batch: {len(probs)} problems, {sum(p.rhs.size for p in probs)} cells, flags {flags}", flush=True)
+        b = api.Batch(list(zip(spec, devs)), flags=flags)
+        try:
+            st = b.solve(stats=True)
+            eager = [d.p.download() for d in devs]
+            for i, (P, ref, (sw, res)) in enumerate(zip(eager, refs, st)):
+                err = _rel(P, ref)
+                assert err <= 1e-12 and res <= 1e-11 and sw == 0, (i, spec[i], err, res)
+            for rep in range(2):
+                for d in devs:
+                    d.p.fill_zero()
+                api.sync()
+                b.solve()
+                api.sync()
+                for i, (d, P0) in enumerate(zip(devs, eager)):
+                    assert np.array_equal(d.p.download(), P0), f"replay {rep} differs (problem {i})"
+        finally:
+            b.destroy()
+    finally:
+        for d in devs:
+            d.free()
+
+
+def test_batch_c4_distribution_matches_oracle():
+    """24 problems of the 96-problem workload's distribution through the
+    per-problem graph branches."""
+    _c4_batch_parity(24, 11, api.PPF_BATCH_NO_TUNE)
+
+
+@pytest.mark.skip(reason="the fused batch launch (one persistent kernel over a device work queue, "
+                         "ppf_fused.hpp) did not complete in either of its two GPU runs and the cause "
+                         "is not shown; the path is switched off (kFusedBatch in ppf_batch.hip)")
+def test_fused_batch_matches_oracle():
+    """The fused launch on 24 problems of the 96-problem workload's
+    distribution, to the same bar as the per-problem branches."""
+    _c4_batch_parity(24, 11)
+
+
 def test_tune_picks_a_variant_and_keeps_parity():
     prob = W.smoothed_normal((64, 64, 96), (False, False, False), 1.5, seed=9)
     ax, ay, az = prob.ax
