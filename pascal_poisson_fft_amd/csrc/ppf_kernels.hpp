@@ -931,19 +931,17 @@ k_zthomas(T *__restrict__ W, T *__restrict__ cs, ZArgs z, const double *__restri
 #endif
 // FP32: the hardware reciprocal is correct to about 1 ulp
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
-// reciprocal: hardware estimate refined by NR Newton steps (each doubles the
-// correct bits; two give about 1 ulp, one is far below FP32 rounding)
-template <int NR>
-__device__ __forceinline__ double frcp_nr(double x) {
+__device__ __forceinline__ double frcp(double x) {
+    // reciprocal: hardware estimate refined by Newton steps (each doubles the
+    // correct bits; two give about 1 ulp)
     double r = __builtin_amdgcn_rcp(x);
 #pragma unroll
-    for (int i = 0; i < NR; ++i) {
+    for (int i = 0; i < PPF_RCP_NEWTON; ++i) {
         const double e = __builtin_fma(-x, r, 1.0);
         r = __builtin_fma(r, e, r);
     }
     return r;
 }
-__device__ __forceinline__ double frcp(double x) { return frcp_nr<PPF_RCP_NEWTON>(x); }
 
 // WPE: minimum waves per SIMD the register allocation must allow (occupancy
 // is what lets one workgroup's loads overlap another's arithmetic here).
@@ -958,13 +956,10 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     const int n3m = z.n3m;
     constexpr int KS = (K + S - 1) / S;           // table rows per thread
     const int NPAD = KS * (int)blockDim.x;        // >= NP: every thread writes KS entries
-    // row table: (a_g, c_g) and bz_g = -a_g - c_g, so b_g = (bz_g - lx) - ly;
-    // padding rows past n3m are (0, 0), bz = -1: decoupled, zero right-hand
-    // side, so their x is 0 like the identity rows they stand for
-    double2 *acz = reinterpret_cast<double2 *>(zsh);  // [NPAD]
-    double *bzt = zsh + 2 * NPAD;                     // [NPAD]
+    double *am = zsh;               // [NPAD], 0 past n3m
+    double *ap = am + NPAD;         // [NPAD], 0 past n3m
     const int PS = P * S;
-    CT *ra = reinterpret_cast<CT *>(bzt + NPAD);  // [3][P][S]: first-row equations, then reduction buffer
+    CT *ra = reinterpret_cast<CT *>(ap + NPAD);  // [3][P][S]: first-row equations, then reduction buffer
     CT *rc = ra + 3 * PS;           // [3][P][S]: reduction buffer
     CT *xe = rc + 3 * PS;           // [P][S]: partition end values x_{K-1}
     CT *ps = ra;                    // [P][S]: partial sums, once the reduction is done
@@ -978,11 +973,8 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     T *w = W + (size_t)jp * z.Lxs + (active ? kx : 0);
     const CT lx = active ? (CT)z.lamx[kx] : CT(0);
     const CT ly = (CT)z.lamy[jp];
+    const bool pin = active && z.zx[kx] && z.zy[jp];
     const int g0 = p * K;
-    // the (0,0) mode's first row is pinned (identity, zero right-hand side):
-    // only row 0 of partition 0 can be it
-    const bool pin0 = active && p == 0 && z.zx[kx] && z.zy[jp];
-    const int nlive = active ? n3m - g0 : 0;  // rows l < nlive carry data
     // loads are clamped in bounds and selected, so the sweep code is branch-free
     const int glast = n3m - 1;
     // Row coefficients (the per-solve table filled by pass 1) are loaded
@@ -1013,8 +1005,8 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
 #pragma unroll
     for (int j = 0; j < KS; ++j) {
         const int k = tid + j * (int)blockDim.x;
-        acz[k] = k < n3m ? double2{ca[j], cp[j]} : double2{0.0, 0.0};
-        bzt[k] = k < n3m ? -ca[j] - cp[j] : -1.0;
+        am[k] = k < n3m ? ca[j] : 0.0;
+        ap[k] = k < n3m ? cp[j] : 0.0;
     }
     CT bc[4];
 #pragma unroll
@@ -1022,23 +1014,18 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     __syncthreads();
     // a row's value is masked where it is first used: zero past n3m, in
     // inactive columns, and in the pinned first row of the (0,0) mode
-    auto dmask = [&](int l) { d[l] = (l < nlive && !(l == 0 && pin0)) ? d[l] : CT(0); };
-    // rows 1..K-1 from the table; row 0 (reverse sweep only) may be pinned
+    auto dmask = [&](int l) {
+        const int g = g0 + l;
+        d[l] = (active && g < n3m && !(pin && g == 0)) ? d[l] : CT(0);
+    };
+    // identity rows: padding past n3m, and the pinned first row of the (0,0) mode
     auto coef = [&](int l, CT &a, CT &b, CT &c) {
         const int g = g0 + l;
-        const double2 t = acz[g];
-        a = (CT)t.x;
-        c = (CT)t.y;
-        b = ((CT)bzt[g] - lx) - ly;
-        if (l == 0) {
-            c = pin0 ? CT(0) : c;
-            b = pin0 ? CT(1) : b;
-        }
-    };
-    // FP32 workspaces: one Newton step (the solve is rounded to FP32 anyway)
-    auto rcp = [](CT x) -> CT {
-        if constexpr (sizeof(CT) == 8) return frcp_nr<(sizeof(T) == 4 ? 1 : PPF_RCP_NEWTON)>(x);
-        else return frcp(x);
+        const bool id = (g >= n3m) || (pin && g == 0);
+        const CT am_ = (CT)am[g], ap_ = (CT)ap[g];
+        a = am_;
+        c = id ? CT(0) : ap_;
+        b = id ? CT(1) : (-am_ - ap_ - lx) - ly;
     };
     // Reverse sweep (first-row equation) and forward sweep (last-row
     // equation) interleaved: two independent dependency chains per step.
@@ -1050,24 +1037,23 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
         dmask(K - 2);
         dmask(1);
         coef(K - 2, a, b, c);
-        CT r = rcp(b);
+        double r = frcp(b);
         ah = r * a; ch = r * c; dh = r * d[K - 2];
         coef(1, a, b, c);
-        r = rcp(b);
+        r = frcp(b);
         rr[1] = r;
         Af = r * a; Cf = r * c; Df = r * d[1];
 #pragma unroll
         for (int i = 1; i <= K - 2; ++i) {
             const int lb = K - 2 - i;  // reverse row
             const int lf = 1 + i;      // forward row
-            // each row is masked at its first visit (rows 0 and K-1 have one)
-            if (2 * i < K - 3 || lb == 0) dmask(lb);
-            if (2 * i < K - 3 || lf == K - 1) dmask(lf);
+            dmask(lb);
+            dmask(lf);
             CT a1, b1, c1, a2, b2, c2;
             coef(lb, a1, b1, c1);
             coef(lf, a2, b2, c2);
-            const CT r1 = rcp(b1 - c1 * ah);
-            const CT r2 = rcp(b2 - a2 * Cf);
+            const double r1 = frcp(b1 - c1 * ah);
+            const double r2 = frcp(b2 - a2 * Cf);
             ch = -r1 * c1 * ch;
             dh = r1 * (d[lb] - c1 * dh);
             ah = r1 * a1;
@@ -1099,7 +1085,7 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
             ch1 = ra[PS + o + S];
             dh1 = ra[2 * PS + o + S];
         }
-        const CT inv = rcp(CT(1) - Af * ch - Cf * ah1);
+        const CT inv = frcp(CT(1) - Af * ch - Cf * ah1);
         el = -Af * ah * inv;
         eu = -Cf * ch1 * inv;
         er = (Df - Af * dh - Cf * dh1) * inv;
@@ -1124,7 +1110,7 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
                 const int q = o + h * S;
                 lp = buf[q]; up = buf[PS + q]; rp = buf[2 * PS + q];
             }
-            const CT inv = rcp(CT(1) - el * um - eu * lp);
+            const CT inv = frcp(CT(1) - el * um - eu * lp);
             er = (er - el * rm - eu * rp) * inv;
             el = -el * lm * inv;
             eu = -eu * up * inv;

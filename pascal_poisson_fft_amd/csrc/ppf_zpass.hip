@@ -37,7 +37,7 @@ int launch_z(ppf_plan *p, T *W, const double *dx3, const double *dmx3, hipStream
         constexpr int NTMAX = decltype(nt)::value, WPE = decltype(wpe)::value;
         const int P = (n3m + K - 1) / K;
         constexpr int KS = (K + S - 1) / S;  // z table rows per thread
-        const size_t shm = sizeof(double) * (3 * (size_t)KS * S * P + 7 * (size_t)P * S);
+        const size_t shm = sizeof(double) * (2 * (size_t)KS * S * P + 7 * (size_t)P * S);
         if (P < 1 || S * P > NTMAX || shm > 65536) return;
         const int tiles = (p->Lx + S - 1) / S;
         const int ntiles = tiles * p->znjr;
