@@ -271,10 +271,10 @@ int fused_residual(ppf_fused *f, int i, hipStream_t s, double *rel) {
     xr.ih2 = 1.0 / (h2 * h2);
     xr.pbc1 = d.pbc1 != 0;
     xr.pbc2 = d.pbc2 != 0;
-    constexpr int KZ = 32;
-    const long ntiles = (long)((d.n1m + 255) / 256) * d.n2m * ((d.n3m + KZ - 1) / KZ);
+    const int ty = residual_rows(d.n1m, d.n2m, d.n3m, 4L * num_cus());
+    const long ntiles = residual_tiles(d.n1m, d.n2m, d.n3m, ty);
     const int g = (int)std::min<long>(std::min<long>(kResPartials, 8L * num_cus()), ntiles);
-    hipLaunchKernelGGL(k_residual<KZ>, dim3(g), dim3(256), 0, s, q.in, xr, d.n1m, d.n2m, d.n3m, nullptr);
+    launch_k_residual(ty, g, s, q.in, xr, d.n1m, d.n2m, d.n3m, nullptr);
     hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, s, f->partial, g, f->red, nullptr);
     PPF_HIP(hipGetLastError());
     double hr[3];

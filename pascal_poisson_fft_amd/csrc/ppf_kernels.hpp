@@ -1407,32 +1407,55 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
 // FP64 residual r = f - L p on interior cells of the ghosted P (the operator
 // of XRes), with per-workgroup sums of r^2, f^2 and p (deterministic
 // two-level reduction with k_reduce3); with xr.rout set, r is also stored in
-// FP32 in the W row pass 1 transforms next (the refinement sweep's input).  A
-// tile is 256 consecutive x cells of one row j over KZ consecutive planes;
-// each thread walks its column up the planes keeping p(k-1), p(k), p(k+1) in
-// registers, so each plane of P is loaded once per column (its x/y
-// neighbours come from the caches).  Workgroups stride over the tiles.
-// (Giving each XCD consecutive tiles, so that y neighbours meet in one L2,
-// halved the fetched bytes and still measured slower: 0.70 vs 0.65 ms.)
+// FP32 in the W row pass 1 transforms next (the refinement sweep's input).
+// A tile is 256 consecutive x cells of TY consecutive rows j (TY divides n2m)
+// over KZ consecutive planes.  Each thread walks its TY columns up the planes
+// keeping p(k-1), p(k), p(k+1) of all TY rows in registers:
+//  * y neighbours inside the block come from those registers; only the two
+//    halo rows (j0-1 and j0+TY, or the wall row itself, or the periodic
+//    wrap) are loaded;
+//  * x neighbours come from the adjacent lanes of the wavefront (DPP wave
+//    shifts); only a wave's edge lanes and the domain's edge cells load them;
+//  * the tiles of one XCD (workgroup b runs on XCD b mod 8) are consecutive,
+//    so a halo row is usually in that XCD's L2 already.
+// P is fetched about 1.1 times at 512^3 (one row per thread, with x and y
+// neighbours loaded, fetched it about 2.7 times: 35 against 16 B per cell,
+// 0.70 against 0.49 ms).  Workgroups stride over the tiles.
 // ---------------------------------------------------------------------------
-template <int KZ>
+// value of lane l-1 (l+1) of the wavefront: DPP wave_shr:1 (wave_shl:1) per dword
+__device__ __forceinline__ double dpp_lane_minus1(double v) {
+    const int2 w = __builtin_bit_cast(int2, v);
+    const int2 r{__builtin_amdgcn_update_dpp(0, w.x, 0x138, 0xf, 0xf, false),
+                 __builtin_amdgcn_update_dpp(0, w.y, 0x138, 0xf, 0xf, false)};
+    return __builtin_bit_cast(double, r);
+}
+__device__ __forceinline__ double dpp_lane_plus1(double v) {
+    const int2 w = __builtin_bit_cast(int2, v);
+    const int2 r{__builtin_amdgcn_update_dpp(0, w.x, 0x130, 0xf, 0xf, false),
+                 __builtin_amdgcn_update_dpp(0, w.y, 0x130, 0xf, 0xf, false)};
+    return __builtin_bit_cast(double, r);
+}
+
+template <int KZ, int TY>
 __global__ void __launch_bounds__(256)
 k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m, const int *__restrict__ gate) {
     if (gated_off(gate)) return;
     __shared__ double red[256];
-    const int nxc = (n1m + 255) / 256, nkb = (n3m + KZ - 1) / KZ;
-    const long ntiles = (long)nxc * n2m * nkb;
+    const int nxc = (n1m + 255) / 256, nyb = n2m / TY, nkb = (n3m + KZ - 1) / KZ;
+    const long ntiles = (long)nxc * nyb * nkb;
     const long L1 = (long)n1m + 2, L2 = (long)n2m + 2, pl = L1 * L2;
     const double *__restrict__ P = xr.P;
     double sr = 0.0, sf = 0.0, sx = 0.0;
-    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // XCD-grouped tile order (a permutation of the workgroups when 8 | grid)
+    const long vb = (gridDim.x % 8 == 0) ? (long)(blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8 : blockIdx.x;
+    for (long tile = vb; tile < ntiles; tile += gridDim.x) {
         const int xc = (int)(tile % nxc);
         const long rest = tile / nxc;
-        const int j = (int)(rest % n2m), kb = (int)(rest / n2m);
+        const int jb = (int)(rest % nyb), kb = (int)(rest / nyb);
         const int i = xc * 256 + (int)threadIdx.x;
         if (i >= n1m) continue;
-        // neighbour offsets: zero-flux walls reuse the centre, periodic axes wrap
-        long dxm = -1, dxp = 1, dym = -L1, dyp = L1;
+        // x neighbour offsets: zero-flux walls reuse the centre, periodic x wraps
+        long dxm = -1, dxp = 1;
         if (xr.pbc1) {
             if (i == 0) dxm = n1m - 1;
             if (i == n1m - 1) dxp = -(long)(n1m - 1);
@@ -1440,41 +1463,60 @@ k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m, con
             if (i == 0) dxm = 0;
             if (i == n1m - 1) dxp = 0;
         }
-        if (xr.pbc2) {
-            if (j == 0) dym = (long)(n2m - 1) * L1;
-            if (j == n2m - 1) dyp = -(long)(n2m - 1) * L1;
-        } else {
-            if (j == 0) dym = 0;
-            if (j == n2m - 1) dyp = 0;
-        }
+        const int lane = (int)threadIdx.x & 63;
+        const bool xedge_m = lane == 0 || i == 0, xedge_p = lane == 63 || i == n1m - 1;
+        // halo rows of the block: the row below j0 and the row above j0+TY-1
+        // (at a wall the edge row itself, for periodic y the wrapped row)
+        const int j0 = jb * TY;
+        const int jlo = j0 > 0 ? j0 - 1 : (xr.pbc2 ? n2m - 1 : 0);
+        const int jhi = j0 + TY < n2m ? j0 + TY : (xr.pbc2 ? 0 : n2m - 1);
+        const long dlo = (long)(jlo - j0) * L1, dhi = (long)(jhi - j0) * L1;
         const int k0 = kb * KZ;
-        const long pc0 = ((long)(k0 + 1) * L2 + (j + 1)) * L1 + (i + 1);
-        double zm = P[pc0 - pl], c = P[pc0];  // ghost plane below the first (finite, weight 0 at the wall)
-        // planes past n3m load the last plane again (clamped addresses, no
-        // branch), so all KZ planes' loads issue together; their terms are
-        // masked out of the sums
+        const long pc0 = ((long)(k0 + 1) * L2 + (j0 + 1)) * L1 + (i + 1);
+        double zm[TY], c[TY];  // planes k-1 and k of the TY rows
 #pragma unroll
+        for (int t = 0; t < TY; ++t) {
+            zm[t] = P[pc0 + t * L1 - pl];  // ghost plane below the first (finite, weight 0 at the wall)
+            c[t] = P[pc0 + t * L1];
+        }
+        // planes past n3m load the last plane again (clamped addresses, no
+        // branch); their terms are masked out of the sums
+#pragma unroll 2
         for (int kk = 0; kk < KZ; ++kk) {
             const int k0k = k0 + kk;
             const bool in = k0k < n3m;
             const int k = in ? k0k : n3m - 1;
             const long pc = pc0 + (long)(k - k0) * pl;
-            const double zp = P[pc + pl];
-            const double xm = P[pc + dxm], xp = P[pc + dxp], ym = P[pc + dym], yp = P[pc + dyp];
-            const double fv = ld_stream(&f[((size_t)k * n2m + j) * n1m + i]);
+            double zp[TY];
+#pragma unroll
+            for (int t = 0; t < TY; ++t) zp[t] = P[pc + t * L1 + pl];
+            const double hlo = P[pc + dlo], hhi = P[pc + dhi];
             const double am = xr.zam[k], ap = xr.zap[k];
-            const double lp = (xp - 2.0 * c + xm) * xr.ih1 + (yp - 2.0 * c + ym) * xr.ih2 + am * zm - (am + ap) * c +
-                              ap * zp;
-            const double r = fv - lp;
-            if (in) {
-                sr += r * r;
-                sf += fv * fv;
-                sx += c;
-                if (xr.rout)
-                    st_stream(&xr.rout[((size_t)(k + 1) * (size_t)xr.Ly + (size_t)(j + 1)) * (size_t)xr.Lxs + i], (float)r);
+#pragma unroll
+            for (int t = 0; t < TY; ++t) {
+                const long q = pc + t * L1;
+                double xm = dpp_lane_minus1(c[t]), xp = dpp_lane_plus1(c[t]);
+                if (xedge_m) xm = P[q + dxm];
+                if (xedge_p) xp = P[q + dxp];
+                const double ym = t > 0 ? c[t - 1] : hlo, yp = t < TY - 1 ? c[t + 1] : hhi;
+                const double fv = ld_stream(&f[((size_t)k * n2m + (j0 + t)) * n1m + i]);
+                const double lp = (xp - 2.0 * c[t] + xm) * xr.ih1 + (yp - 2.0 * c[t] + ym) * xr.ih2 + am * zm[t] -
+                                  (am + ap) * c[t] + ap * zp[t];
+                const double r = fv - lp;
+                if (in) {
+                    sr += r * r;
+                    sf += fv * fv;
+                    sx += c[t];
+                    if (xr.rout)
+                        st_stream(&xr.rout[((size_t)(k + 1) * (size_t)xr.Ly + (size_t)(j0 + t + 1)) * (size_t)xr.Lxs + i],
+                                  (float)r);
+                }
             }
-            zm = c;
-            c = zp;
+#pragma unroll
+            for (int t = 0; t < TY; ++t) {
+                zm[t] = c[t];
+                c[t] = zp[t];
+            }
         }
     }
     const double tr = block_sum<256>(sr, red);
@@ -1484,6 +1526,30 @@ k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m, con
         xr.partial[3 * blockIdx.x] = tr;
         xr.partial[3 * blockIdx.x + 1] = tf;
         xr.partial[3 * blockIdx.x + 2] = tx;
+    }
+}
+
+// Planes per residual tile, and the tile count for TY rows per tile.
+constexpr int kResKZ = 32;
+inline long residual_tiles(int n1m, int n2m, int n3m, int ty) {
+    return (long)((n1m + 255) / 256) * (n2m / ty) * ((n3m + kResKZ - 1) / kResKZ);
+}
+// Rows per residual tile: the largest of 8, 4, 2 dividing n2m that still
+// leaves min_tiles tiles (small problems need the parallelism more than the
+// halo saving), else 1.
+inline int residual_rows(int n1m, int n2m, int n3m, long min_tiles) {
+    for (int ty = 8; ty > 1; ty >>= 1)
+        if (n2m % ty == 0 && residual_tiles(n1m, n2m, n3m, ty) >= min_tiles) return ty;
+    return 1;
+}
+// The launch (grid g <= tiles, one partial triple per workgroup).
+inline void launch_k_residual(int ty, int g, hipStream_t s, const double *f, const XRes &xr, int n1m, int n2m, int n3m,
+                              const int *gate) {
+    switch (ty) {
+        case 8: hipLaunchKernelGGL((k_residual<kResKZ, 8>), dim3(g), dim3(256), 0, s, f, xr, n1m, n2m, n3m, gate); break;
+        case 4: hipLaunchKernelGGL((k_residual<kResKZ, 4>), dim3(g), dim3(256), 0, s, f, xr, n1m, n2m, n3m, gate); break;
+        case 2: hipLaunchKernelGGL((k_residual<kResKZ, 2>), dim3(g), dim3(256), 0, s, f, xr, n1m, n2m, n3m, gate); break;
+        default: hipLaunchKernelGGL((k_residual<kResKZ, 1>), dim3(g), dim3(256), 0, s, f, xr, n1m, n2m, n3m, gate); break;
     }
 }
 

@@ -227,10 +227,10 @@ static int launch_residual(ppf_plan *p, const double *f, const double *P, hipStr
         xr.Ly = p->Ly;
         xr.Lxs = p->Lxs;
     }
-    constexpr int KZ = 32;  // planes per tile (4..32 and 8 or 16 workgroups per CU measured alike)
-    const long ntiles = (long)((p->d.n1m + 255) / 256) * p->d.n2m * ((n3m + KZ - 1) / KZ);
+    const int ty = residual_rows(p->d.n1m, p->d.n2m, n3m, 4L * num_cus());
+    const long ntiles = residual_tiles(p->d.n1m, p->d.n2m, n3m, ty);
     const int g = (int)std::min<long>(std::min<long>(p->npartial, 8L * num_cus()), ntiles);
-    hipLaunchKernelGGL(k_residual<KZ>, dim3(g), dim3(256), 0, s, f, xr, p->d.n1m, p->d.n2m, n3m, p->gate);
+    launch_k_residual(ty, g, s, f, xr, p->d.n1m, p->d.n2m, n3m, p->gate);
     hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, s, p->partial, g, p->red, p->gate);
     PPF_HIP(hipGetLastError());
     return PPF_OK;

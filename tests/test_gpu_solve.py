@@ -88,16 +88,45 @@ def test_c1_manufactured_64():
     assert O.rel_residual(prob.rhs, got, prob.L, prob.pbc, az.dmx, az.dx) < 1e-11
 
 
-def test_gpu_residual_matches_oracle():
-    ax, ay, az, f = _problem(32, 24, 16, (True, False), False, 1.2, 3)
-    P = np.random.default_rng(4).standard_normal((18, 26, 34))
-    plan = api.FFTPoisson(32, 24, 16, (ax.L, ay.L, az.L), (True, False, False))
+@pytest.mark.parametrize("pbc", BCS, ids=["NN", "PP", "PN", "NP"])
+@pytest.mark.parametrize("n1,n2,n3", [(32, 24, 16), (96, 20, 40), (320, 10, 34), (160, 28, 6), (64, 12, 70)])
+def test_gpu_residual_matches_oracle(pbc, n1, n2, n3):
+    """The residual kernel on a random iterate: y blocks of 8, 4 and 2 rows
+    (n2m = 24; 20, 12, 28; 10 -- extents are even, so never 1), one and two x
+    tiles with the last cell inside a wave (n1m = 32, 160), z blocks shorter
+    than 32 planes, every x/y branch.
+    A wrong neighbour in any row or column of cells moves the sum of r^2 by
+    far more than the 1e-12 bound."""
+    ax, ay, az, f = _problem(n1, n2, n3, pbc, False, 1.2, 3)
+    P = np.random.default_rng(4).standard_normal((n3 + 2, n2 + 2, n1 + 2))
+    plan = api.FFTPoisson(n1, n2, n3, (ax.L, ay.L, az.L), (pbc[0], pbc[1], False))
     fd, pd = api.DeviceArray.from_host(f), api.DeviceArray.from_host(P)
     d3, x3 = api.DeviceArray.from_host(az.dmx), api.DeviceArray.from_host(az.dx)
     r = plan.residual(fd, pd, d3, x3)
-    ref = O.rel_residual(f, P, (ax.L, ay.L, az.L), (True, False), az.dmx, az.dx)
-    assert abs(r - ref) <= 1e-12 * ref
+    ref = O.rel_residual(f, P, (ax.L, ay.L, az.L), pbc, az.dmx, az.dx)
+    assert abs(r - ref) <= 1e-12 * ref, (r, ref)
     plan.clean()
+
+
+@pytest.mark.parametrize("pbc", [(False, False), (True, True)], ids=["NN", "PP"])
+@pytest.mark.parametrize("n1,n2,n3", [(512, 256, 512), (512, 256, 256), (512, 256, 128)])
+def test_gpu_residual_row_blocks_match_oracle(pbc, n1, n2, n3):
+    """Shapes large enough that the residual kernel blocks 8, 4 and 2 rows per
+    thread (it keeps at least 4 tiles per CU on a 256-CU device: 512 x 256 x
+    512, 256, 128 cells give 8, 4, 2)."""
+    ax, ay, az, f = _problem(n1, n2, n3, pbc, False, 1.2, 5)
+    P = np.random.default_rng(6).standard_normal((n3 + 2, n2 + 2, n1 + 2))
+    plan = api.FFTPoisson(n1, n2, n3, (ax.L, ay.L, az.L), (pbc[0], pbc[1], False))
+    fd, pd = api.DeviceArray.from_host(f), api.DeviceArray.from_host(P)
+    d3, x3 = api.DeviceArray.from_host(az.dmx), api.DeviceArray.from_host(az.dx)
+    try:
+        r = plan.residual(fd, pd, d3, x3)
+    finally:
+        plan.clean()
+        for a in (fd, pd, d3, x3):
+            a.free()
+    ref = O.rel_residual(f, P, (ax.L, ay.L, az.L), pbc, az.dmx, az.dx)
+    assert abs(r - ref) <= 1e-12 * ref, (r, ref)
 
 
 def test_repeated_solves_are_deterministic_and_leave_rhs_intact():
