@@ -378,9 +378,9 @@ def main(argv=None):
     rel_res = plan.residual(dev.rhs, dev.p, dev.dmx3, dev.dx3)
     p64 = dev.p.download()  # the FP64 solution, for the oracle comparison
     # FP64 certification residual (the 1e-10 check the FP64 path needs too),
-    # timed where it runs: right after a solve (pairs of solve + residual; the
-    # residual synchronises and reads back 24 bytes).  Back to back on a
-    # settled field it measures about 0.2 ms less.
+    # timed as a caller pays for it: pairs of solve + residual, the residual
+    # synchronising and reading back its 24-byte sums, reported as the
+    # increment over the solve.
     pairs = max(3, a.steps // 2)
     api.sync()
     t0 = time.perf_counter()
