@@ -26,6 +26,11 @@ LIB_PATH = os.path.join(_PKG, "lib", "libpascal_poisson_fft.so")
 
 PPF_PREC_FP64 = 0
 PPF_PREC_FP32_IR = 1
+PPF_ERR_INVALID_ARG = -1
+PPF_ERR_SIZE = -2
+PPF_ERR_HIP = -3
+PPF_ERR_ALLOC = -4
+PPF_ERR_UNSUPPORTED = -5
 PPF_ERR_NOT_CONVERGED = -6
 PPF_ERR_SCRATCH = -7
 
