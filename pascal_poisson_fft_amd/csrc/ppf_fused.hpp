@@ -23,6 +23,12 @@
 // (invalidating its CU's and XCD's caches).  An acquire on every poll would
 // invalidate the whole L2 each time; with hundreds of workgroups waiting at a
 // stage boundary that starves the very tiles they wait for.
+//
+// Status: switched off (kFusedBatch in ppf_batch.hip).  The launch did not
+// complete on the GPU (24 problems of the 96-problem workload), and the
+// cause has not been shown: the waits are bounded, the barriers are reached
+// by every thread and the ticket counter only grows.  Batches run one plan
+// per problem as graph branches instead.
 #pragma once
 
 #include "ppf_kernels.hpp"
