@@ -310,6 +310,34 @@ def stream_leg(api, W, nsteps: int = 200, n=(1024, 1024, 768)):
     return out
 
 
+def rocfft_crosscheck(n: int, stage_ms):
+    """Timing cross-check against the vendor FFT: hipFFT (rocFFT) batched FP64
+    real transforms of the n*n x-lines of n points, the transform alone (the
+    reference adds DCT pre/post passes around it).  The solve path uses no
+    vendor FFT; the cross-check library is built apart from it."""
+    import ctypes as C
+    path = os.path.join(ROOT, "pascal_poisson_fft_amd", "lib", "libppf_rocfft_xcheck.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.ppfx_rocfft_lines.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    fwd, inv = C.c_float(), C.c_float()
+    rc = L.ppfx_rocfft_lines(n, n * n, 10, C.byref(fwd), C.byref(inv))
+    if rc != 0:
+        return {"error": rc}
+    real_b, cplx_b = 8.0 * n ** 3, 16.0 * (n // 2 + 1) * n * n
+    stages = dict(zip(("x-forward", "y-forward", "z-thomas", "y-inverse", "x-inverse"), stage_ms))
+    return {
+        "what": f"hipFFT/rocFFT batched FP64 D2Z and Z2D of {n * n} lines of {n} points (transform only), "
+                "HIP events, mean of 10 launches",
+        "d2z_ms": round(fwd.value, 4), "z2d_ms": round(inv.value, 4),
+        "d2z_gbs": round((real_b + cplx_b) / (fwd.value * 1e-3) / 1e9, 1),
+        "z2d_gbs": round((real_b + cplx_b) / (inv.value * 1e-3) / 1e9, 1),
+        "solver_x_forward_ms": round(float(stages["x-forward"]), 4),
+        "solver_x_inverse_ms": round(float(stages["x-inverse"]), 4),
+    }
+
+
 def load_traffic(kernel_key: str):
     """HBM bytes per launch of the dominant pass from committed rocprofv3 PMC
     summaries (profiles/traffic.json), or None."""
@@ -437,6 +465,7 @@ def main(argv=None):
                            "tuning_s": round(tune_s, 2)},
         "rel_residual": rel_res,
         "fp32_refine_vs_fp64": prec,
+        "rocfft_crosscheck": rocfft_crosscheck(n, stage_ms),
     }
     if not a.no_batch:
         print("# batch leg (96 problems)", file=sys.stderr, flush=True)

@@ -163,10 +163,12 @@ int ppf_plan_tune(ppf_plan *plan, const double *prhs, double *p, const double *d
  * makes later solves return PPF_ERR_UNSUPPORTED. */
 int ppf_plan_set_zvariant(ppf_plan *plan, int zvariant);
 
-/* The plan's pass variants: variants[0] x pass (0/1), [1] y pass (0/1), [2] z
- * shape (as ppf_plan_set_zvariant). */
+/* The plan's pass variants: variants[0] x pass (0: one butterfly per thread
+ * per stage, 1: half the threads with two, 2: as 0 without holding the next
+ * tile in registers), [1] y pass (0/1 as for x), [2] z shape (as
+ * ppf_plan_set_zvariant). */
 int ppf_plan_variants(const ppf_plan *plan, int variants[3]);
-/* Pins all three (x and y: 0 or 1). */
+/* Pins all three (x: 0..2, y: 0 or 1). */
 int ppf_plan_set_variants(ppf_plan *plan, int xvariant, int yvariant, int zvariant);
 
 /* ---- batched solves of independent problems ----------------------------- */

@@ -81,6 +81,25 @@ def build_fortran() -> None:
         raise RuntimeError(f"amdflang failed:\n{r.stderr}")
 
 
+XCHECK_SRC = os.path.join(PKG, "xcheck", "rocfft_xcheck.hip")
+XCHECK_LIB = "libppf_rocfft_xcheck.so"
+
+
+def build_xcheck() -> str | None:
+    """The vendor-FFT timing cross-check (bench only; the solve library does
+    not link it).  Best effort: a missing hipFFT skips it."""
+    if not os.path.exists(XCHECK_SRC):
+        return None
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, XCHECK_LIB)
+    cmd = [HIPCC, *HIPFLAGS, "-shared", XCHECK_SRC, "-o", out, "-lhipfft"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        print(f"rocFFT cross-check not built:\n{r.stderr[-2000:]}", file=sys.stderr)
+        return None
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib-only", action="store_true")
@@ -90,6 +109,7 @@ def main(argv=None):
     print("built", out)
     if not a.lib_only:
         build_fortran()
+        build_xcheck()
 
 
 if __name__ == "__main__":

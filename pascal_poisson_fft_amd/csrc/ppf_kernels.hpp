@@ -107,10 +107,17 @@ struct LdsLines {
 // 1: half the threads per line, two butterflies each (twice the lines per
 // workgroup).
 // ---------------------------------------------------------------------------
+// XV: 0 one radix-R butterfly per thread per stage; 1 half the threads, two
+// each; 2 as 0 without the next tile's prefetch held in registers across the
+// transform (fewer registers; faster on 512-point FP32 lines, slower on 1024).
+#ifndef PPF_XLITE_WPE
+#define PPF_XLITE_WPE 4
+#endif
 template <typename T, int N, bool PER, int XV = 0>
 struct XGeom {
     using G = LineGeom<N, PER>;
-    static constexpr int TPL = (G::TPL >> XV) > 0 ? (G::TPL >> XV) : 1;
+    static constexpr bool LITE = XV == 2;
+    static constexpr int TPL = (G::TPL >> (XV == 1 ? 1 : 0)) > 0 ? (G::TPL >> (XV == 1 ? 1 : 0)) : 1;
     static constexpr int LPB = (256 / TPL) > 0 ? (256 / TPL) : 1;
     static constexpr int NT = LPB * TPL;
     using L = LdsLines<T, G::M, G::STRIDE, LPB>;
@@ -118,7 +125,8 @@ struct XGeom {
     // (very short or 1024-point lines) admits fewer workgroups per CU
     static constexpr int WG_LDS = (160 * 1024) / (L::SIZE * (int)sizeof(T));
     static constexpr int WPE_LDS = (WG_LDS * (NT / 64)) / 4 > 0 ? (WG_LDS * (NT / 64)) / 4 : 1;
-    static constexpr int WPE = PPF_XWPE < WPE_LDS ? PPF_XWPE : WPE_LDS;
+    static constexpr int WPE_REG = LITE ? PPF_XLITE_WPE : PPF_XWPE;
+    static constexpr int WPE = WPE_REG < WPE_LDS ? WPE_REG : WPE_LDS;
 };
 
 // q = a / b for 0 <= a < 2^24 via a float reciprocal and one correction
@@ -244,7 +252,7 @@ k_xfwd(const double *__restrict__ in, T *W, int nlines, int n2m, int Ly,
         }
     };
     int tile = blockIdx.x;
-    if (tile < ntiles) fetch(tile);
+    if (!X::LITE && tile < ntiles) fetch(tile);
     // the z row table of this solve, for pass 3 (saves a launch)
     if (zr.zam && blockIdx.x == 0) fill_z_rows(zr, tid, NT);
     L::load_tables(lds, tw, wk, qk, tid, NT);
@@ -252,6 +260,7 @@ k_xfwd(const double *__restrict__ in, T *W, int nlines, int n2m, int Ly,
     const LLine<T> my = L::line(lds, l);
     for (; tile < ntiles; tile += gridDim.x) {
         __syncthreads();  // the previous tile's LDS lines have been stored
+        if (X::LITE) fetch(tile);
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = tid + q * NT;
@@ -266,7 +275,7 @@ k_xfwd(const double *__restrict__ in, T *W, int nlines, int n2m, int Ly,
                 }
             }
         }
-        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+        if (!X::LITE && tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
         __syncthreads();
         fft_forward<T, M, TPL>(my, t, tab);
         if (PER)
@@ -359,7 +368,7 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
         }
     };
     int tile = blockIdx.x;
-    if (tile < ntiles) fetch(tile);
+    if (!X::LITE && tile < ntiles) fetch(tile);
     L::load_tables(lds, tw, wk, qk, tid, NT);
     // ACC: the iterate's own mean (sum from the residual pass) goes too, so
     // the refined solution keeps the zero mean exactly (the residual is blind
@@ -380,6 +389,7 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
     const LLine<T> my = L::line(lds, l);
     for (; tile < ntiles; tile += gridDim.x) {
         __syncthreads();  // the previous tile's LDS lines have been stored
+        if (X::LITE) fetch(tile);
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = tid + q * NT;
@@ -393,7 +403,7 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
 #pragma unroll
             for (int q = 0; q < EPTP; ++q) pcur[q] = pold[q];
         }
-        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+        if (!X::LITE && tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
         __syncthreads();
         if (PER)
             c2r_pre<T, N, TPL>(my, t, tab + M);

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <array>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -429,16 +430,15 @@ int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1
         }
     }
     const int keep[3] = {p->xvariant, p->yvariant, p->zvariant};
-    // mean stage times of 2 timed solves after 1 untimed; UNSUPPORTED if the
-    // current shapes do not fit the extents
+    // per-stage minimum over 4 timed solves after 1 untimed (variants differ
+    // by a few percent, about the run-to-run spread of one solve);
+    // UNSUPPORTED if the current shapes do not fit the extents
     auto time_stages = [&](float out[5]) -> int {
-        for (int i = 0; i < 5; ++i) out[i] = 0.f;
-        for (int rep = 0; rep < 3; ++rep) {
+        for (int rep = 0; rep < 5; ++rep) {
             float st[5];
             const int rc = ppf_solve_timed(p, prhs, P, dmx1, dmx2, dmx3, dx3, stream, st);
             if (rc) return rc;
-            if (rep > 0)
-                for (int i = 0; i < 5; ++i) out[i] += 0.5f * st[i];
+            for (int i = 0; i < 5; ++i) out[i] = rep == 1 ? st[i] : (rep > 1 && st[i] < out[i] ? st[i] : out[i]);
         }
         return PPF_OK;
     };
@@ -453,14 +453,14 @@ int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1
     float bx = 0.f, by = 0.f;
     int vx = 0, vy = 0;
     p->zvariant = keep[2];
-    for (int v = 0; v < kXYVariants; ++v) {
-        p->xvariant = v;
-        p->yvariant = v;
+    for (int v = 0; v < std::max(kXVariants, kYVariants); ++v) {
+        p->xvariant = v < kXVariants ? v : 0;
+        p->yvariant = v < kYVariants ? v : 0;
         float st[5];
         const int rc = time_stages(st);
         if (rc) return restore(rc);
-        if (v == 0 || st[0] + st[4] < bx) { bx = st[0] + st[4]; vx = v; }
-        if (v == 0 || st[1] + st[3] < by) { by = st[1] + st[3]; vy = v; }
+        if (v < kXVariants && (v == 0 || st[0] + st[4] < bx)) { bx = st[0] + st[4]; vx = v; }
+        if (v < kYVariants && (v == 0 || st[1] + st[3] < by)) { by = st[1] + st[3]; vy = v; }
     }
     p->xvariant = vx;
     p->yvariant = vy;
@@ -494,7 +494,7 @@ int ppf_plan_variants(const ppf_plan *p, int variants[3]) {
 }
 
 int ppf_plan_set_variants(ppf_plan *p, int xvariant, int yvariant, int zvariant) {
-    if (!p || xvariant < 0 || xvariant >= kXYVariants || yvariant < 0 || yvariant >= kXYVariants)
+    if (!p || xvariant < 0 || xvariant >= kXVariants || yvariant < 0 || yvariant >= kYVariants)
         return PPF_ERR_INVALID_ARG;
     const int rc = ppf_plan_set_zvariant(p, zvariant);
     if (rc) return rc;
