@@ -20,9 +20,17 @@ k_zcoef(const double *__restrict__ dx3, const double *__restrict__ dmx3, int n3m
 }
 
 // Totals of the per-workgroup triples (sum r^2, sum f^2, sum x) in a fixed
-// order (deterministic).
+// order (deterministic).  With check set it is also the device-side
+// refinement test (below; check.gate is the gate to test and close, the same
+// one that gates this kernel), which saves a launch per test.
+struct GateTest {
+    int *gate = nullptr;
+    double tol = 0.0;
+    int count = 0;
+    double *res_out = nullptr;
+};
 static __global__ void __launch_bounds__(256) k_reduce3(const double *__restrict__ partial, int n, double *__restrict__ out,
-                                                 const int *__restrict__ gate) {
+                                                 const int *__restrict__ gate, GateTest check = GateTest{}) {
     if (gated_off(gate)) return;
     __shared__ double red[256];
     double a = 0.0, b = 0.0, c = 0.0;
@@ -38,27 +46,25 @@ static __global__ void __launch_bounds__(256) k_reduce3(const double *__restrict
         out[0] = a;
         out[1] = b;
         out[2] = c;
+        if (check.gate && check.gate[0] != 0) {
+            const double rel = b > 0 ? sqrt(a / b) : sqrt(a);
+            *check.res_out = rel;
+            check.gate[1] += check.count;
+            check.gate[0] = (rel <= check.tol) ? 0 : 1;
+        }
     }
 }
 
 // Device-side refinement test.  k_gate_init opens the gate (gate[0] = 1: more
-// sweeps may run) and sets the sweep count gate[1]; k_gate_check, while the
-// gate is open, reads the totals of the last residual reduction (sum r^2,
-// sum f^2), stores ||r||/||f|| in res_out, counts the sweep that produced it
-// (count = 1 after a gated sweep) and closes the gate once the residual is
-// <= tol (a NaN residual keeps it open, so the sweeps run out and the caller
-// sees the miss).  One thread each.
+// sweeps may run) and sets the sweep count gate[1]; the test (k_reduce3 with
+// a GateTest, while the gate is open) stores ||r||/||f|| of the totals it
+// just reduced in res_out, counts the sweep that produced them (count = 1
+// after a gated sweep) and closes the gate once the residual is <= tol (a
+// NaN residual keeps it open, so the sweeps run out and the caller sees the
+// miss).
 static __global__ void k_gate_init(int *__restrict__ gate, int sweeps) {
     gate[0] = 1;
     gate[1] = sweeps;
-}
-static __global__ void k_gate_check(const double *__restrict__ red, double tol, int count, int *__restrict__ gate,
-                             double *__restrict__ res_out) {
-    if (gate[0] == 0) return;
-    const double rel = red[1] > 0 ? sqrt(red[0] / red[1]) : sqrt(red[0]);
-    *res_out = rel;
-    gate[1] += count;
-    gate[0] = (rel <= tol) ? 0 : 1;
 }
 
 // Ghost layers of P from its interior, one face pair per launch:

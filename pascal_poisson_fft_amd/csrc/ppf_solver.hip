@@ -220,7 +220,11 @@ static XRes make_xres(const ppf_plan *p, const double *P) {
 // FP64 residual of P against f into p->red (sum r^2, sum f^2, sum p), from
 // the z row table already in p->zcoef; gated by p->gate.  out (FP32 plans):
 // r is also stored in W32, where the refinement form of pass 1 reads it.
-static int launch_residual(ppf_plan *p, const double *f, const double *P, hipStream_t s, bool out = false) {
+// reduce = false skips the totals (a store right after a test of the same P,
+// whose totals are already in p->red); test folds the device-side
+// refinement test into the reduction.
+static int launch_residual(ppf_plan *p, const double *f, const double *P, hipStream_t s, bool out = false,
+                           bool reduce = true, const GateTest &test = GateTest{}) {
     const int n3m = p->d.n3m;
     XRes xr = make_xres(p, P);
     if (out) {
@@ -232,7 +236,7 @@ static int launch_residual(ppf_plan *p, const double *f, const double *P, hipStr
     const long ntiles = residual_tiles(p->d.n1m, p->d.n2m, n3m, ty);
     const int g = (int)std::min<long>(std::min<long>(p->npartial, 8L * num_cus()), ntiles);
     launch_k_residual(ty, g, s, f, xr, p->d.n1m, p->d.n2m, n3m, p->gate);
-    hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, s, p->partial, g, p->red, p->gate);
+    if (reduce) hipLaunchKernelGGL(k_reduce3, dim3(1), dim3(256), 0, s, p->partial, g, p->red, p->gate, test);
     PPF_HIP(hipGetLastError());
     return PPF_OK;
 }
@@ -311,7 +315,7 @@ static int solve_fp32_ir(ppf_plan *p, const double *f, double *P, const double *
             rel = p->hred[1] > 0 ? std::sqrt(p->hred[0] / p->hred[1]) : std::sqrt(p->hred[0]);
             if (rel <= tol) { converged = true; break; }
             if (sweeps == maxs || !(rel == rel)) break;
-            if (!store && (rc = launch_residual(p, f, P, s, true))) return rc;
+            if (!store && (rc = launch_residual(p, f, P, s, true, false))) return rc;
             if ((rc = correct32(p, P, dmx1, dmx2, dmx3, dx3, s))) return rc;
             ++sweeps;
         }
@@ -326,9 +330,9 @@ static int solve_fp32_ir(ppf_plan *p, const double *f, double *P, const double *
 
 // FP32_IR solve with the convergence test on the device, for callers that
 // cannot synchronise (graph capture, graph replays of new right-hand sides).
-// min_sweeps corrections always run; then a residual test (k_residual +
-// k_gate_check) decides on the device whether each of up to extra_sweeps
-// gated corrections runs (extra_sweeps < 0: up to max_sweeps in all).  Once
+// min_sweeps corrections always run; then a residual test (k_residual and
+// the gate test in its reduction) decides on the device whether each of up to
+// extra_sweeps gated corrections runs (extra_sweeps < 0: up to max_sweeps in all).  Once
 // the residual is <= tol every later gated kernel returns at once -- but is
 // still a launch, which is why a captured graph keeps extra_sweeps small.
 // Only kernel launches: no host synchronisation, no memset.
@@ -353,10 +357,11 @@ static int solve_fp32_async(ppf_plan *p, const double *f, double *P, const doubl
     // feeds the gated correction (both return at once once the gate closed)
     for (int i = nmin;; ++i) {
         p->gate = p->dgate;  // open at the first test
-        if ((rc = launch_residual(p, f, P, s, false))) return rc;
-        hipLaunchKernelGGL(k_gate_check, dim3(1), dim3(1), 0, s, p->red, tol, i > nmin ? 1 : 0, p->dgate, p->dres);
+        if ((rc = launch_residual(p, f, P, s, false, true, GateTest{p->dgate, tol, i > nmin ? 1 : 0, p->dres})))
+            return rc;
         if (i >= nmax) break;
-        if ((rc = launch_residual(p, f, P, s, true)) || (rc = correct32(p, P, dmx1, dmx2, dmx3, dx3, s))) return rc;
+        if ((rc = launch_residual(p, f, P, s, true, false)) || (rc = correct32(p, P, dmx1, dmx2, dmx3, dx3, s)))
+            return rc;
     }
     p->gate = nullptr;
     if ((rc = refresh_ghosts(p, P, dmx1, dmx2, dmx3, s))) return rc;
