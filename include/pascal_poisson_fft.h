@@ -165,10 +165,11 @@ int ppf_plan_set_zvariant(ppf_plan *plan, int zvariant);
 
 /* The plan's pass variants: variants[0] x pass (0: one butterfly per thread
  * per stage, 1: half the threads with two, 2: as 0 without holding the next
- * tile in registers), [1] y pass (0/1 as for x), [2] z shape (as
+ * tile in registers, 3: as 2 with one workgroup per tile instead of a
+ * persistent grid), [1] y pass (0/1 as for x), [2] z shape (as
  * ppf_plan_set_zvariant). */
 int ppf_plan_variants(const ppf_plan *plan, int variants[3]);
-/* Pins all three (x: 0..2, y: 0 or 1). */
+/* Pins all three (x: 0..3, y: 0 or 1). */
 int ppf_plan_set_variants(ppf_plan *plan, int xvariant, int yvariant, int zvariant);
 
 /* ---- batched solves of independent problems ----------------------------- */
@@ -254,8 +255,9 @@ int ppf_device_count(int *count);
 int ppf_set_device(int device);
 
 /* ---- measurement helper (no counterpart in the reference) --------------- */
-/* Streams `count` doubles (count even, 16-B aligned buffers) from src to dst
- * `reps` times with a plain copy kernel on `stream` and returns the mean
+/* Streams `count` doubles (count even and > 0, 16-B aligned buffers) from src
+ * to dst `reps` times with a streaming copy kernel (one nontemporal 16-byte
+ * load and store per thread) on `stream` and returns the mean
  * duration of one copy in *ms (HIP events; synchronises).  The measured
  * bandwidth 16*count/ms is the yardstick the pass kernels are compared with,
  * and the known byte count calibrates rocprofv3's HBM byte counters. */

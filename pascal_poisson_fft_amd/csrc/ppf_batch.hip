@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <new>
 #include <tuple>
@@ -138,18 +139,21 @@ int ppf_batch_create(int nprob, const ppf_desc *descs, const ppf_batch_io *io, i
     }
     // one autotuner run per distinct (extents, BCs, precision)
     if (!rc && !b->fused && !(flags & PPF_BATCH_NO_TUNE)) {
-        std::map<std::tuple<int, int, int, int, int, int>, int> tuned;
+        // every problem of a shape gets the x, y and z variants tuned on its first
+        std::map<std::tuple<int, int, int, int, int, int>, std::array<int, 3>> tuned;
         for (int i = 0; i < nprob && !rc; ++i) {
             const ppf_desc &d = descs[i];
             const auto key = std::make_tuple(d.n1m, d.n2m, d.n3m, d.pbc1 != 0, d.pbc2 != 0, d.precision);
             auto it = tuned.find(key);
             if (it == tuned.end()) {
                 int v = 0;
+                std::array<int, 3> vs{};
                 const ppf_batch_io &q = io[i];
                 rc = ppf_plan_tune(b->plan[i], q.prhs, q.p, q.dmx1, q.dmx2, q.dmx3, q.dx3, nullptr, &v);
-                tuned[key] = v;
+                if (!rc) rc = ppf_plan_variants(b->plan[i], vs.data());
+                tuned[key] = vs;
             } else {
-                rc = ppf_plan_set_zvariant(b->plan[i], it->second);
+                rc = ppf_plan_set_variants(b->plan[i], it->second[0], it->second[1], it->second[2]);
             }
         }
     }

@@ -251,8 +251,9 @@ k_xfwd(const double *__restrict__ in, T *W, int nlines, int n2m, int Ly,
             }
         }
     };
+    // the first tile's loads (every form) overlap the set-up below
     int tile = blockIdx.x;
-    if (!X::LITE && tile < ntiles) fetch(tile);
+    if (tile < ntiles) fetch(tile);
     // the z row table of this solve, for pass 3 (saves a launch)
     if (zr.zam && blockIdx.x == 0) fill_z_rows(zr, tid, NT);
     L::load_tables(lds, tw, wk, qk, tid, NT);
@@ -260,7 +261,7 @@ k_xfwd(const double *__restrict__ in, T *W, int nlines, int n2m, int Ly,
     const LLine<T> my = L::line(lds, l);
     for (; tile < ntiles; tile += gridDim.x) {
         __syncthreads();  // the previous tile's LDS lines have been stored
-        if (X::LITE) fetch(tile);
+        if (X::LITE && tile != (int)blockIdx.x) fetch(tile);
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = tid + q * NT;
@@ -367,8 +368,9 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
             }
         }
     };
+    // the first tile's loads (every form) overlap the set-up below
     int tile = blockIdx.x;
-    if (!X::LITE && tile < ntiles) fetch(tile);
+    if (tile < ntiles) fetch(tile);
     L::load_tables(lds, tw, wk, qk, tid, NT);
     // ACC: the iterate's own mean (sum from the residual pass) goes too, so
     // the refined solution keeps the zero mean exactly (the residual is blind
@@ -389,7 +391,7 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
     const LLine<T> my = L::line(lds, l);
     for (; tile < ntiles; tile += gridDim.x) {
         __syncthreads();  // the previous tile's LDS lines have been stored
-        if (X::LITE) fetch(tile);
+        if (X::LITE && tile != (int)blockIdx.x) fetch(tile);
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = tid + q * NT;

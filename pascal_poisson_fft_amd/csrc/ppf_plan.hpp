@@ -198,13 +198,13 @@ inline EigGeom eig_geom(const ppf_plan *p) {
     return EigGeom{p->d.pbc1 != 0, p->ymode, p->d.n1m, p->d.n2m, p->d.L1 / p->d.n1m, p->d.L2 / p->d.n2m};
 }
 
-
 // z-pass shapes: 1-5 and 12-16 partitioned (k_zpart), 6 one thread per line
 // (k_zthomas), 7-11 and 17 pipelined (k_zpipe); 0 = chosen by n3m.
 constexpr int kZVariants = 17;
 // x-pass variants (XGeom): 0 and 1 threads per line, 2 without the register
-// prefetch; y-pass variants (YGeom): 0 and 1.
-constexpr int kXVariants = 3, kYVariants = 2;
+// prefetch, 3 the same kernel as 2 with one workgroup per tile (not
+// persistent); y-pass variants (YGeom): 0 and 1.
+constexpr int kXVariants = 4, kYVariants = 2;
 
 // The z row table of a plan (filled by pass 1 when passed to dispatch_xfwd).
 inline ZRows z_rows(const ppf_plan *p, const double *dx3, const double *dmx3) {

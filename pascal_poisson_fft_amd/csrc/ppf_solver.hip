@@ -741,10 +741,11 @@ int ppf_device_count(int *count) {
 int ppf_set_device(int device) { return hipSetDevice(device) == hipSuccess ? PPF_OK : PPF_ERR_HIP; }
 
 int ppf_copy_timed(double *dst, const double *src, size_t count, int reps, void *stream, float *ms) {
-    if (!dst || !src || reps <= 0 || !ms || (count & 1)) return PPF_ERR_INVALID_ARG;
+    if (!dst || !src || reps <= 0 || !ms || count == 0 || (count & 1)) return PPF_ERR_INVALID_ARG;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const size_t n2 = count / 2;
-    const unsigned grid = 8u * (unsigned)num_cus();
+    if ((n2 + 255) / 256 > 0x7fffffffu) return PPF_ERR_SIZE;
+    const unsigned grid = (unsigned)((n2 + 255) / 256);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess) return PPF_ERR_HIP;
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -754,8 +755,8 @@ int ppf_copy_timed(double *dst, const double *src, size_t count, int reps, void 
     int rc = PPF_OK;
     if (hipEventRecord(e0, s) != hipSuccess) rc = PPF_ERR_HIP;
     for (int r = 0; r < reps && rc == PPF_OK; ++r) {
-        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, reinterpret_cast<double2 *>(dst),
-                           reinterpret_cast<const double2 *>(src), n2);
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, reinterpret_cast<D2v *>(dst),
+                           reinterpret_cast<const D2v *>(src), n2);
         if (hipGetLastError() != hipSuccess) rc = PPF_ERR_HIP;
     }
     if (rc == PPF_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||

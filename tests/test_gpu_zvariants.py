@@ -113,13 +113,13 @@ def test_long_z_lines_use_one_thread_per_line(pbc):
 def test_xy_variants_match_oracle(pbc, n1, n2, n3):
     """Every x- and y-pass variant the tuner can pick (threads per line: one
     butterfly per stage, or half the threads with two; x also without the
-    register prefetch) against the oracle, at 1e-12, for every boundary
-    branch."""
+    register prefetch, on a persistent grid or one workgroup per tile) against
+    the oracle, at 1e-12, for every boundary branch."""
     L, ax, ay, az, f, ref = _case(n1, n2, n3, pbc, seed=n1 + n2 + n3)
     plan = api.FFTPoisson(n1, n2, n3, L, (pbc[0], pbc[1], False))
     dev = api.Problem3D(f, ax, ay, az)
     try:
-        for xv, yv in ((0, 1), (1, 0), (1, 1), (2, 0), (2, 1)):
+        for xv, yv in ((0, 1), (1, 0), (1, 1), (2, 0), (2, 1), (3, 0), (3, 1)):
             print(f"{n1}x{n2}x{n3} pbc={pbc} xv={xv} yv={yv}", flush=True)
             plan.set_variants(xv, yv, 0)
             assert plan.variants() == (xv, yv, 0)
@@ -133,17 +133,19 @@ def test_xy_variants_match_oracle(pbc, n1, n2, n3):
         plan.clean()
 
 
+@pytest.mark.parametrize("xv", [2, 3])
 @pytest.mark.parametrize("pbc", [(False, False), (True, False)], ids=["NN", "PN"])
-def test_x_variant_without_prefetch_fp32_ir(pbc):
+def test_x_variant_without_prefetch_fp32_ir(pbc, xv):
     """The refinement forms of the x passes (residual read in place, correction
-    added into P) in the variant without the register prefetch: FP32_IR reaches
-    1e-10 and matches the oracle to 1e-10."""
+    added into P) in the variants without the register prefetch (persistent
+    grid, one workgroup per tile): FP32_IR reaches 1e-10 and matches the oracle
+    to 1e-10."""
     n1, n2, n3 = 512, 64, 8
     L, ax, ay, az, f, ref = _case(n1, n2, n3, pbc, seed=7)
     plan = api.FFTPoisson(n1, n2, n3, L, (pbc[0], pbc[1], False), precision=api.PPF_PREC_FP32_IR, tol=1e-10)
     dev = api.Problem3D(f, ax, ay, az)
     try:
-        plan.set_variants(2, 0, 0)
+        plan.set_variants(xv, 0, 0)
         st = plan.solve(dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3)
         api.sync()
         err = float(np.linalg.norm(dev.p.download() - ref) / np.linalg.norm(ref))
@@ -165,7 +167,7 @@ def test_tuner_caches_per_shape_and_keeps_parity():
         z = plans[0].tune(*args)
         v = plans[0].variants()
         print(f"tuned variants {v}", flush=True)
-        assert v[2] == z and v[0] in (0, 1) and v[1] in (0, 1) and z > 0
+        assert v[2] == z and 0 <= v[0] < 4 and v[1] in (0, 1) and z > 0
         assert plans[1].tune(*args) == z and plans[1].variants() == v
         for pl in plans:
             dev.p.fill_zero()
