@@ -1448,7 +1448,7 @@ __device__ __forceinline__ double dpp_lane_plus1(double v) {
     return __builtin_bit_cast(double, r);
 }
 
-template <int KZ, int TY>
+template <int KZ, int TY, bool STORE>
 __global__ void __launch_bounds__(256)
 k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m, const int *__restrict__ gate) {
     if (gated_off(gate)) return;
@@ -1504,6 +1504,9 @@ k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m, con
             for (int t = 0; t < TY; ++t) zp[t] = P[pc + t * L1 + pl];
             const double hlo = P[pc + dlo], hhi = P[pc + dhi];
             const double am = xr.zam[k], ap = xr.zap[k];
+            // STORE: r of row j0+t goes to W32 row (k+1) Ly + j0+t+1
+            float *rrow = nullptr;
+            if constexpr (STORE) rrow = xr.rout + ((size_t)(k + 1) * (size_t)xr.Ly + (size_t)(j0 + 1)) * (size_t)xr.Lxs + i;
 #pragma unroll
             for (int t = 0; t < TY; ++t) {
                 const long q = pc + t * L1;
@@ -1519,9 +1522,7 @@ k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m, con
                     sr += r * r;
                     sf += fv * fv;
                     sx += c[t];
-                    if (xr.rout)
-                        st_stream(&xr.rout[((size_t)(k + 1) * (size_t)xr.Ly + (size_t)(j0 + t + 1)) * (size_t)xr.Lxs + i],
-                                  (float)r);
+                    if constexpr (STORE) st_stream(rrow + (size_t)t * (size_t)xr.Lxs, (float)r);
                 }
             }
 #pragma unroll
@@ -1555,14 +1556,21 @@ inline int residual_rows(int n1m, int n2m, int n3m, long min_tiles) {
     return 1;
 }
 // The launch (grid g <= tiles, one partial triple per workgroup).
+template <bool STORE>
+inline void launch_k_residual_s(int ty, int g, hipStream_t s, const double *f, const XRes &xr, int n1m, int n2m,
+                                int n3m, const int *gate) {
+    switch (ty) {
+        case 8: hipLaunchKernelGGL((k_residual<kResKZ, 8, STORE>), dim3(g), dim3(256), 0, s, f, xr, n1m, n2m, n3m, gate); break;
+        case 4: hipLaunchKernelGGL((k_residual<kResKZ, 4, STORE>), dim3(g), dim3(256), 0, s, f, xr, n1m, n2m, n3m, gate); break;
+        case 2: hipLaunchKernelGGL((k_residual<kResKZ, 2, STORE>), dim3(g), dim3(256), 0, s, f, xr, n1m, n2m, n3m, gate); break;
+        default: hipLaunchKernelGGL((k_residual<kResKZ, 1, STORE>), dim3(g), dim3(256), 0, s, f, xr, n1m, n2m, n3m, gate); break;
+    }
+}
+// r is stored (FP32, into xr.rout) when xr.rout is set
 inline void launch_k_residual(int ty, int g, hipStream_t s, const double *f, const XRes &xr, int n1m, int n2m, int n3m,
                               const int *gate) {
-    switch (ty) {
-        case 8: hipLaunchKernelGGL((k_residual<kResKZ, 8>), dim3(g), dim3(256), 0, s, f, xr, n1m, n2m, n3m, gate); break;
-        case 4: hipLaunchKernelGGL((k_residual<kResKZ, 4>), dim3(g), dim3(256), 0, s, f, xr, n1m, n2m, n3m, gate); break;
-        case 2: hipLaunchKernelGGL((k_residual<kResKZ, 2>), dim3(g), dim3(256), 0, s, f, xr, n1m, n2m, n3m, gate); break;
-        default: hipLaunchKernelGGL((k_residual<kResKZ, 1>), dim3(g), dim3(256), 0, s, f, xr, n1m, n2m, n3m, gate); break;
-    }
+    if (xr.rout) launch_k_residual_s<true>(ty, g, s, f, xr, n1m, n2m, n3m, gate);
+    else launch_k_residual_s<false>(ty, g, s, f, xr, n1m, n2m, n3m, gate);
 }
 
 }  // namespace ppf
