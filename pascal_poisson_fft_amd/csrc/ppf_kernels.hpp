@@ -103,13 +103,12 @@ struct LdsLines {
 // ---------------------------------------------------------------------------
 // pass 1: x-forward.  One LDS line per prhs row; LPB rows per workgroup.
 // PER: periodic x (R2C, Lx = N+2) else DCT-II (Lx = N).
-// XV: the tuner's variant -- 0: one radix-R butterfly per thread per stage;
-// 1: half the threads per line, two butterflies each (twice the lines per
-// workgroup).
+// XV: the kernel form of the tuner's variant -- 0: one radix-R butterfly per
+// thread per stage; 1: half the threads per line, two butterflies each
+// (twice the lines per workgroup); 2: as 0 without the next tile's prefetch
+// held in registers across the transform (fewer registers, more waves).
+// Variant 3 is the XV = 2 kernel launched with one workgroup per tile.
 // ---------------------------------------------------------------------------
-// XV: 0 one radix-R butterfly per thread per stage; 1 half the threads, two
-// each; 2 as 0 without the next tile's prefetch held in registers across the
-// transform (fewer registers; faster on 512-point FP32 lines, slower on 1024).
 #ifndef PPF_XLITE_WPE
 #define PPF_XLITE_WPE 4
 #endif
@@ -199,11 +198,15 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
     return t;
 }
 
-// The line-transform kernels are persistent and software-pipelined: a
-// workgroup walks tiles tile = blockIdx.x, blockIdx.x + gridDim.x, ...; the
-// next tile's global loads are issued into registers right after the current
-// tile has been scattered into LDS, so they are in flight while the current
-// tile is transformed and stored (barriers wait on LDS only).
+// The line-transform kernels walk tiles tile = blockIdx.x, blockIdx.x +
+// gridDim.x, ...  On a persistent grid (as many workgroups as fit) the forms
+// with the prefetch are software-pipelined: the next tile's global loads are
+// issued into registers right after the current tile has been scattered
+// into LDS, so they are in flight while the current tile is transformed and
+// stored (barriers wait on LDS only).  Every form issues its first tile's
+// loads before staging the twiddle tables; on a grid of one workgroup per
+// tile (x variant 3) that is the only overlap inside a workgroup, and the
+// other workgroups on the CU supply the rest.
 //
 // RES: the refinement form -- the line is the residual r = f - L x that
 // k_residual stored in FP32 in this line's own W row (in is unused), and the
