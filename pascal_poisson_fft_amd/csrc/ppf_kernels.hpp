@@ -82,11 +82,16 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
 // (LSD odd, so consecutive lines start on different banks and the
 // column-to-line scatter of the y passes is conflict-free), followed by the
 // axis tables tw[M] | wk[M+1] | qk[M+1], copied from global memory on entry
-// together with the data so no FFT stage waits on a global round trip.
-template <typename T, int M, int S, int NL>
+// together with the data so no FFT stage waits on a global round trip.  QK:
+// the DCT's qk table is held (only DCT lines read it; the real-FFT and
+// complex forms leave it out, which at 1024 points lets a third x-pass
+// workgroup onto the CU).
+template <int M, bool QK>
+constexpr int lds_table_slots() { return M + (M + 1) + (QK ? M + 1 : 0); }
+template <typename T, int M, int S, int NL, bool QK = true>
 struct LdsLines {
     static constexpr int LSD = 2 * lds_slots(S) + PPF_LINE_PAD;  // S complex values + pad scalars
-    static constexpr int TABN = M + 2 * (M + 1);
+    static constexpr int TABN = lds_table_slots<M, QK>();
     static constexpr int SIZE = NL * LSD + 2 * TABN;  // scalars of type T
     __device__ __forceinline__ static LLine<T> line(T *lds, int l) {
         return LLine<T>{reinterpret_cast<cx<T> *>(lds + l * LSD)};
@@ -119,7 +124,7 @@ struct XGeom {
     static constexpr int TPL = (G::TPL >> (XV == 1 ? 1 : 0)) > 0 ? (G::TPL >> (XV == 1 ? 1 : 0)) : 1;
     static constexpr int LPB = (256 / TPL) > 0 ? (256 / TPL) : 1;
     static constexpr int NT = LPB * TPL;
-    using L = LdsLines<T, G::M, G::STRIDE, LPB>;
+    using L = LdsLines<T, G::M, G::STRIDE, LPB, !PER>;
     // register target: PPF_XWPE waves per SIMD, or fewer where the LDS tile
     // (very short or 1024-point lines) admits fewer workgroups per CU
     static constexpr int WG_LDS = (160 * 1024) / (L::SIZE * (int)sizeof(T));
@@ -471,7 +476,8 @@ struct YGeom {
     // LDS tile -- lines + tables -- would exceed the CU's 160 KiB); lines
     // beyond kMaxNT / TPL are transformed in further rounds
     static constexpr int lds_bytes(int tx) {
-        return ((CPLX ? tx / 2 : tx) * (2 * lds_slots(S) + PPF_LINE_PAD) + 2 * (M + 2 * (M + 1))) * (int)sizeof(T);
+        return ((CPLX ? tx / 2 : tx) * (2 * lds_slots(S) + PPF_LINE_PAD) + 2 * lds_table_slots<M, MODE == 0>()) *
+               (int)sizeof(T);
     }
     static constexpr int TX0 = PPF_YROW_BYTES / (int)sizeof(T);
     static constexpr int tx_fit(int tx) { return (tx > 2 && lds_bytes(tx) > 160 * 1024 - 1024) ? tx_fit(tx / 2) : tx; }
@@ -484,7 +490,7 @@ struct YGeom {
     static constexpr int NT = NL / ROUNDS * TPL;
     static_assert(NT <= NTCAP, "y-pass workgroup exceeds its thread cap");
     static_assert(lds_bytes(TX) <= 160 * 1024, "y-pass tile exceeds the CU's LDS");
-    using L = LdsLines<T, M, S, NL>;
+    using L = LdsLines<T, M, S, NL, MODE == 0>;
     // Row-blocked mapping (real columns, DCT or R2C): thread (g, c2) owns the
     // RPT consecutive rows RPT*g .. RPT*g+RPT-1 of column pair c2.  In the
     // DCT's Makhoul order a thread's rows fill 2*(RPT/4) complex LDS slots
