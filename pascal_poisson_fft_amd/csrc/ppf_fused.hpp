@@ -58,7 +58,9 @@ struct FusedProb {
     const double *in;                          // prhs (n1m, n2m, n3m)
     double *P;                                 // (n1m+2)(n2m+2)(n3m+2)
     const double *dmx1, *dmx2, *dmx3, *dx3;
-    double *W;                                 // workspace (Lxs, Ly, Lz)
+    double *W;                                 // workspace (Lxs, Ly, Lz), row-major: private to
+                                               // this kernel's own stages, not the tiled layout
+                                               // of the per-problem passes
     const cx<double> *xtw, *xwk, *xqk, *ytw, *ywk, *yqk;
     const double *lamx, *lamy;
     const unsigned char *zx, *zy;

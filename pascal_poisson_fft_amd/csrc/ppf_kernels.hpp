@@ -1,8 +1,14 @@
 // ppf_kernels.hpp -- the five passes of the solve for gfx950.
 //
-// Workspace W (type T) holds one spectral field laid out x fastest:
-//   W[(kp * Ly + jp) * Lxs + i],  i < Lx = n1m (Neumann x) or n1m+2 (periodic x,
-//   half spectrum as interleaved re/im), Ly = n2m+2, Lz = n3m+2.
+// Workspace W (type T) holds one spectral field as Lz planes of Ly rows of
+// Lxs scalars (i < Lx = n1m (Neumann x) or n1m+2 (periodic x, half spectrum
+// as interleaved re/im), Lxs = Lx rounded up to 128 bytes, Ly = n2m+2,
+// Lz = n3m+2).  Inside a plane the rows are cut into 128-byte column tiles
+// of WT = 128/sizeof(T) scalars, each tile a contiguous block of Ly rows:
+//   W[kp * Ly * Lxs + (i / WT) * Ly * WT + jp * WT + i % WT]   (w_at)
+// so the y passes' column strips and the x passes' 8-line stores are
+// runs of consecutive bytes (a strip of the row-major layout is Ly segments
+// of 128 bytes, Lxs apart).
 // The forward passes use rows jp = 1..n2m and planes kp = 1..n3m; the z pass
 // adds the two spectral ghost planes kp = 0, n3m+1; the y-inverse fills the
 // y ghost rows jp = 0, n2m+1; the x-inverse writes every row of P (with the x
@@ -26,6 +32,19 @@ constexpr int kMaxNT = 512;
 // on a cache-line boundary (a periodic-x row of n1m+2 scalars otherwise
 // straddles two lines in every tile segment).
 __host__ __device__ constexpr int w_stride(int lx, int tbytes) { return (lx * tbytes + 127) / 128 * 128 / tbytes; }
+
+// Column tiles of W: WT scalars (128 bytes) wide, Ly rows tall, contiguous.
+template <typename T>
+__host__ __device__ constexpr int w_tile() { return 128 / (int)sizeof(T); }
+// offset of column c (row 0) inside a plane of Ly rows
+template <typename T>
+__host__ __device__ inline size_t w_col(int c, int Ly) {
+    constexpr int WT = w_tile<T>();
+    return (size_t)(c / WT) * (size_t)Ly * WT + (size_t)(c % WT);
+}
+// offset of (row j, column c) inside a plane; rows of one column are WT apart
+template <typename T>
+__host__ __device__ inline size_t w_at(int j, int c, int Ly) { return w_col<T>(c, Ly) + (size_t)j * w_tile<T>(); }
 
 // Gate of the device-side refinement test: a pass launched with a gate runs
 // only while gate[0] is nonzero (more sweeps needed); nullptr = always.  The
@@ -88,9 +107,9 @@ __device__ __forceinline__ void st_stream(T *p, T v) {
 // workgroup onto the CU).
 template <int M, bool QK>
 constexpr int lds_table_slots() { return M + (M + 1) + (QK ? M + 1 : 0); }
-template <typename T, int M, int S, int NL, bool QK = true>
+template <typename T, int M, int S, int NL, bool QK = true, int PAD = PPF_LINE_PAD>
 struct LdsLines {
-    static constexpr int LSD = 2 * lds_slots(S) + PPF_LINE_PAD;  // S complex values + pad scalars
+    static constexpr int LSD = 2 * lds_slots(S) + PAD;  // S complex values + pad scalars
     static constexpr int TABN = lds_table_slots<M, QK>();
     static constexpr int SIZE = NL * LSD + 2 * TABN;  // scalars of type T
     __device__ __forceinline__ static LLine<T> line(T *lds, int l) {
@@ -117,6 +136,16 @@ struct LdsLines {
 #ifndef PPF_XLITE_WPE
 #define PPF_XLITE_WPE 4
 #endif
+// Pad of an x-pass LDS line: the line stride is 128 bytes modulo 256, so in
+// the column-tile order of the W accesses (8 lanes of 16 bytes per line,
+// consecutive lines next) the two lines of one 16-lane LDS pass fall on
+// disjoint banks.
+template <typename T>
+constexpr int x_line_pad(int s) {
+    const int base = (2 * lds_slots(s) * (int)sizeof(T)) % 256;
+    return ((128 - base + 256) % 256) / (int)sizeof(T);
+}
+
 template <typename T, int N, bool PER, int XV = 0>
 struct XGeom {
     using G = LineGeom<N, PER>;
@@ -124,7 +153,11 @@ struct XGeom {
     static constexpr int TPL = (G::TPL >> (XV == 1 ? 1 : 0)) > 0 ? (G::TPL >> (XV == 1 ? 1 : 0)) : 1;
     static constexpr int LPB = (256 / TPL) > 0 ? (256 / TPL) : 1;
     static constexpr int NT = LPB * TPL;
-    using L = LdsLines<T, G::M, G::STRIDE, LPB, !PER>;
+    // the bank pad where the padded tile still fits the CU's LDS
+    static constexpr int PADX = x_line_pad<T>(G::STRIDE);
+    static constexpr bool PAD_FITS =
+        LPB * (2 * lds_slots(G::STRIDE) + PADX) + 2 * lds_table_slots<G::M, !PER>() <= (160 * 1024) / (int)sizeof(T);
+    using L = LdsLines<T, G::M, G::STRIDE, LPB, !PER, PAD_FITS ? PADX : PPF_LINE_PAD>;
     // register target: PPF_XWPE waves per SIMD, or fewer where the LDS tile
     // (very short or 1024-point lines) admits fewer workgroups per CU
     static constexpr int WG_LDS = (160 * 1024) / (L::SIZE * (int)sizeof(T));
@@ -229,33 +262,72 @@ k_xfwd(const double *__restrict__ in, T *W, int nlines, int n2m, int Ly,
     constexpr int LX = PER ? N + 2 : N;
     constexpr int LXS = w_stride(LX, (int)sizeof(T));  // W row stride
     constexpr int H = N / 2, HX = LX / 2;          // sample pairs per line / per W row
-    constexpr int EPT = (LPB * H + NT - 1) / NT;  // pair loads per thread per tile
+    // W-side element order: PT pairs fill one 128-byte row of a column tile;
+    // consecutive lanes take the PT pairs of one line, then the same pairs
+    // of the next line, so a wave covers consecutive bytes of a tile.  The
+    // prhs side (row-major lines) keeps line-major order.
+    constexpr int PT = w_tile<T>() / 2;
+    constexpr int HR = RES ? (H + PT - 1) / PT * PT : H;  // pairs per line in the load order
+    constexpr int EPT = (LPB * HR + NT - 1) / NT;        // pair loads per thread per tile
     using D2 = double __attribute__((ext_vector_type(2)));
     using V2 = T __attribute__((ext_vector_type(2)));
     __shared__ __align__(16) T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int ntiles = (nlines + LPB - 1) / LPB;
     const float rn2m = 1.0f / (float)n2m;
+    constexpr int WT = w_tile<T>();
+    constexpr bool ONE_ROW = NT % (PT * LPB) == 0;    // W side: thread -> one line per tile
+    constexpr int HSTEP = NT / LPB;                   // its pair step (ONE_ROW)
+    const int h0 = (tid / (PT * LPB)) * PT + tid % PT;  // its first pair (ONE_ROW)
+    // element e of a tile's loads -> (line li of the tile, pair h)
+    auto emap = [&](int e, int &li, int &h) {
+        if constexpr (RES) {
+            const int r = e / PT;
+            li = r % LPB;
+            h = (r / LPB) * PT + (e - r * PT);
+        } else {
+            li = e / H;
+            h = e - li * H;
+        }
+    };
     D2 pre[EPT];
     // unconditional 16-byte loads of sample pairs from clamped addresses
     // (lines past the end are transformed but never stored); nothing
     // consumes them before the next tile, so they stay in flight
+    // W base of line (j,k) of this tile: plane k+1, row j+1
+    auto wline = [&](int line) -> T * {
+        const int k = fdiv_small(line, n2m, rn2m);
+        return W + (size_t)(k + 1) * Ly * LXS + (size_t)(line - k * n2m + 1) * WT;
+    };
     auto fetch = [&](int tile) {
+        if constexpr (RES && ONE_ROW) {
+            int line = tile * LPB + (tid / PT) % LPB;
+            line = line < nlines ? line : nlines - 1;
+            const T *base = wline(line);
+            const T *p0 = base + w_col<T>(2 * h0, Ly), *plast = base + w_col<T>(2 * (H - 1), Ly);
+            const size_t qstep = (size_t)(HSTEP / PT) * Ly * WT;
 #pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            const int e0 = tid + q * NT;
-            const int e = e0 < LPB * H ? e0 : LPB * H - 1;
-            const int li = e / H, n = 2 * (e - li * H);
-            const int line0 = tile * LPB + li;
-            const int line = line0 < nlines ? line0 : nlines - 1;
-            if constexpr (RES) {
-                // W row of line (j,k): line + Ly + 1 + 2k (as stored below)
-                const int k = fdiv_small(line, n2m, rn2m);
-                const V2 v = ld_stream(reinterpret_cast<const V2 *>(
-                    &W[((size_t)line + (size_t)(Ly + 1) + 2 * (size_t)k) * LXS + n]));
+            for (int q = 0; q < EPT; ++q) {
+                const T *a = h0 + q * HSTEP < H ? p0 + q * qstep : plast;
+                const V2 v = ld_stream(reinterpret_cast<const V2 *>(a));
                 pre[q] = D2{(double)v.x, (double)v.y};
-            } else {
-                pre[q] = ld_stream(reinterpret_cast<const D2 *>(&in[(size_t)line * N + n]));
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                const int e0 = tid + q * NT;
+                const int e = e0 < LPB * HR ? e0 : LPB * HR - 1;
+                int li, h;
+                emap(e, li, h);
+                const int n = 2 * (h < H ? h : H - 1);
+                const int line0 = tile * LPB + li;
+                const int line = line0 < nlines ? line0 : nlines - 1;
+                if constexpr (RES) {
+                    const V2 v = ld_stream(reinterpret_cast<const V2 *>(wline(line) + w_col<T>(n, Ly)));
+                    pre[q] = D2{(double)v.x, (double)v.y};
+                } else {
+                    pre[q] = ld_stream(reinterpret_cast<const D2 *>(&in[(size_t)line * N + n]));
+                }
             }
         }
     };
@@ -273,8 +345,16 @@ k_xfwd(const double *__restrict__ in, T *W, int nlines, int n2m, int Ly,
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = tid + q * NT;
-            if (e < LPB * H) {
-                const int li = e / H, h = e - li * H;
+            int li, h;
+            if constexpr (RES && ONE_ROW) {
+                li = (tid / PT) % LPB;
+                h = h0 + q * HSTEP;
+            } else {
+                li = 0;
+                h = H;
+                if (e < LPB * HR) emap(e, li, h);
+            }
+            if (h < H) {
                 const D2 v = pre[q];
                 if (PER) {  // samples 2h, 2h+1 are complex h of the packed line
                     L::line(lds, li)[h] = cx<T>{(T)v.x, (T)v.y};
@@ -291,22 +371,33 @@ k_xfwd(const double *__restrict__ in, T *W, int nlines, int n2m, int Ly,
             r2c_post<T, N, TPL>(my, t, tab + M);
         else
             dct_post<T, N, TPL>(my, t, tab + M, tab + 2 * M + 1);
-        // W row of prhs line (j,k): (k+1)*Ly + j + 1 = line + n2m + 3 + 2k.
-        // The R2C row's Nyquist pair (h = N/2) is stored apart, so the main
-        // loop covers N/2 pairs per line (no ragged last round).
+        // prhs line (j,k) goes to W plane k+1, row j+1, in the tile order
+        // above.  The R2C row's Nyquist pair (h = N/2) is stored apart, so the
+        // main loop covers N/2 pairs per line.
         constexpr int HM = PER ? HX - 1 : HX;
+        constexpr int HMR = (HM + PT - 1) / PT * PT;
         auto store_pair = [&](int li, int h) {
             const int line = tile * LPB + li;
             if (line < nlines) {
-                const int k = fdiv_small(line, n2m, rn2m);
                 const cx<T> z = L::line(lds, li)[h];
-                st_stream(reinterpret_cast<V2 *>(&W[((size_t)line + (size_t)(Ly + 1) + 2 * (size_t)k) * LXS + 2 * h]),
-                          V2{z.x, z.y});
+                st_stream(reinterpret_cast<V2 *>(wline(line) + w_col<T>(2 * h, Ly)), V2{z.x, z.y});
             }
         };
-        for (int e = tid; e < LPB * HM; e += NT) {
-            const int li = e / HM;
-            store_pair(li, e - li * HM);
+        if constexpr (ONE_ROW) {
+            const int li = (tid / PT) % LPB, line = tile * LPB + li;
+            if (line < nlines) {
+                T *a = wline(line) + w_col<T>(2 * h0, Ly);
+                const size_t qstep = (size_t)(HSTEP / PT) * Ly * WT;
+                for (int h = h0; h < HM; h += HSTEP, a += qstep) {
+                    const cx<T> z = L::line(lds, li)[h];
+                    st_stream(reinterpret_cast<V2 *>(a), V2{z.x, z.y});
+                }
+            }
+        } else {
+            for (int e = tid; e < LPB * HMR; e += NT) {
+                const int r = e / PT, li = r % LPB, h = (r / LPB) * PT + (e - r * PT);
+                if (h < HM) store_pair(li, h);
+            }
         }
         if (PER && tid < LPB) store_pair(tid, HX - 1);
     }
@@ -332,7 +423,7 @@ k_xfwd(const double *__restrict__ in, T *W, int nlines, int n2m, int Ly,
 #endif
 template <typename T, int N, bool PER, bool ACC = false, int XV = 0>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ACC ? (N >= PPF_XIR_CAP_N ? PPF_XACC_WPE : 1) : XGeom<T, N, PER, XV>::WPE)))
-k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
+k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines, int Ly,
        const cx<T> *__restrict__ tw, const cx<T> *__restrict__ wk, const cx<T> *__restrict__ qk,
        const double *__restrict__ dmx1, const double *__restrict__ mean, const double *__restrict__ xsum,
        double inv_cells, const int *__restrict__ gate) {
@@ -345,25 +436,61 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
     constexpr int LXS = w_stride(LX, (int)sizeof(T));  // W row stride
     constexpr int LP = N + 2;
     constexpr int HX = LX / 2, HP = LP / 2;  // scalar pairs per W row / per P row
-    constexpr int EPT = (LPB * HX + NT - 1) / NT;
+    // W rows are read in column-tile order (PT pairs of one row's tile, then
+    // the same pairs of the next row), as k_xfwd stores them
+    constexpr int PT = w_tile<T>() / 2;
+    constexpr int HXR = (HX + PT - 1) / PT * PT;
+    constexpr int EPT = (LPB * HXR + NT - 1) / NT;
     constexpr int EPTP = (LPB * HP + NT - 1) / NT;  // P pairs stored per thread per tile
     using V2 = T __attribute__((ext_vector_type(2)));
     using D2 = double __attribute__((ext_vector_type(2)));
     __shared__ __align__(16) T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int ntiles = (nlines + LPB - 1) / LPB;
+    const float rly = 1.0f / (float)Ly;
+    auto emap = [&](int e, int &li, int &h) {
+        const int r = e / PT;
+        li = r % LPB;
+        h = (r / LPB) * PT + (e - r * PT);
+    };
+    constexpr int WT = w_tile<T>();
+    constexpr bool ONE_ROW = NT % (PT * LPB) == 0;    // thread -> one row per tile
+    constexpr int HSTEP = NT / LPB;                   // its pair step (ONE_ROW)
+    const int h0 = (tid / (PT * LPB)) * PT + tid % PT;  // its first pair (ONE_ROW)
     V2 pre[EPT];
     D2 pold[ACC ? EPTP : 1];
-    // 16-byte (FP64) pair loads of whole rows, clamped to the last pair
+    // 16-byte (FP64) pair loads of whole rows, clamped to the last pair;
+    // row r of W is plane r / Ly, row r % Ly.  When NT is a multiple of
+    // PT * LPB a thread keeps one row for the whole tile (its row, plane
+    // and base address are computed once) and steps its pair by NT / LPB.
     auto fetch = [&](int tile) {
-#pragma unroll
-        for (int q = 0; q < EPT; ++q) {
-            int e = tid + q * NT;
-            e = e < LPB * HX ? e : LPB * HX - 1;
-            const int li = e / HX, h = e - li * HX;
-            int row = tile * LPB + li;
+        if constexpr (ONE_ROW) {
+            int row = tile * LPB + (tid / PT) % LPB;
             row = row < nlines ? row : nlines - 1;
-            pre[q] = ld_stream(reinterpret_cast<const V2 *>(W + (size_t)row * LXS) + h);
+            const int pl = fdiv_small(row, Ly, rly);
+            const T *base = W + (size_t)pl * Ly * LXS + (size_t)(row - pl * Ly) * WT;
+            // pair h0 + q HSTEP is HSTEP/PT column tiles further each step
+            const T *p0 = base + w_col<T>(2 * h0, Ly), *plast = base + w_col<T>(2 * (HX - 1), Ly);
+            const size_t qstep = (size_t)(HSTEP / PT) * Ly * WT;
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                const T *a = h0 + q * HSTEP < HX ? p0 + q * qstep : plast;
+                pre[q] = ld_stream(reinterpret_cast<const V2 *>(a));
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                int e = tid + q * NT;
+                e = e < LPB * HXR ? e : LPB * HXR - 1;
+                int li, h;
+                emap(e, li, h);
+                h = h < HX ? h : HX - 1;
+                int row = tile * LPB + li;
+                row = row < nlines ? row : nlines - 1;
+                const int pl = fdiv_small(row, Ly, rly);
+                pre[q] = ld_stream(
+                    reinterpret_cast<const V2 *>(W + (size_t)pl * Ly * LXS + w_at<T>(row - pl * Ly, 2 * h, Ly)));
+            }
         }
         if constexpr (ACC) {
 #pragma unroll
@@ -403,8 +530,12 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines,
 #pragma unroll
         for (int q = 0; q < EPT; ++q) {
             const int e = tid + q * NT;
-            if (e < LPB * HX) {
-                const int li = e / HX, h = e - li * HX;
+            int li = (tid / PT) % LPB, h = h0 + q * HSTEP;
+            if constexpr (!ONE_ROW) {
+                h = HX;
+                if (e < LPB * HXR) emap(e, li, h);
+            }
+            if (h < HX) {
                 L::line(lds, li)[h] = cx<T>{pre[q].x, pre[q].y};
             }
         }
@@ -522,6 +653,7 @@ k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
     constexpr int TP = TX / 2;  // column pairs: global accesses are 2 adjacent scalars
     constexpr int EPT = (TP * N + NT - 1) / NT;
     using V2 = T __attribute__((ext_vector_type(2)));
+    constexpr int WT = w_tile<T>();  // W row step inside a column tile
     __shared__ __align__(16) T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int nct = (Lx + TX - 1) / TX;
@@ -535,9 +667,9 @@ k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
         if constexpr (Y::RB) {
             int col = ct * TX + 2 * bc2;
             col = col < Lx ? col : Lx - 2;  // Lx is even
-            const T *src = &W[(plane + 1 + (size_t)RPT * bg) * Lxs + col];
+            const T *src = &W[plane * Lxs + w_at<T>(1 + (size_t)RPT * bg, col, Ly)];
 #pragma unroll
-            for (int q = 0; q < RPT; ++q) pre[q] = ld_stream(reinterpret_cast<const V2 *>(src + (size_t)q * Lxs));
+            for (int q = 0; q < RPT; ++q) pre[q] = ld_stream(reinterpret_cast<const V2 *>(src + (size_t)q * WT));
         } else {
 #pragma unroll
             for (int q = 0; q < EPT; ++q) {
@@ -546,7 +678,7 @@ k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
                 const int r = e / TP, c2 = e - r * TP;
                 int col = ct * TX + 2 * c2;
                 col = col < Lx ? col : Lx - 2;  // Lx is even
-                pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[(plane + 1 + r) * Lxs + col]));
+                pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[plane * Lxs + w_at<T>(1 + r, col, Ly)]));
             }
         }
     };
@@ -614,17 +746,17 @@ k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
         const int col = c0 + 2 * bc2;
         if (col < Lx) {
             const LLine<T> c0l = L::line(lds, 2 * bc2), c1l = L::line(lds, 2 * bc2 + 1);
-            T *dst = &W[(plane + R0 + (size_t)RPT * bg) * Lxs + col];
+            T *dst = &W[plane * Lxs + w_at<T>(R0 + (size_t)RPT * bg, col, Ly)];
 #pragma unroll
             for (int k = 0; k < RPT / 2; ++k) {
                 const cx<T> z0 = c0l[(RPT / 2) * bg + k], z1 = c1l[(RPT / 2) * bg + k];
-                st_stream(reinterpret_cast<V2 *>(dst + (size_t)(2 * k) * Lxs), V2{z0.x, z1.x});
-                st_stream(reinterpret_cast<V2 *>(dst + (size_t)(2 * k + 1) * Lxs), V2{z0.y, z1.y});
+                st_stream(reinterpret_cast<V2 *>(dst + (size_t)(2 * k) * WT), V2{z0.x, z1.x});
+                st_stream(reinterpret_cast<V2 *>(dst + (size_t)(2 * k + 1) * WT), V2{z0.y, z1.y});
             }
             if (MODE == 2 && bg == Y::G - 1) {  // R2C: spectral rows N, N+1 (slot N/2)
                 const cx<T> z0 = c0l[N / 2], z1 = c1l[N / 2];
-                st_stream(reinterpret_cast<V2 *>(dst + (size_t)RPT * Lxs), V2{z0.x, z1.x});
-                st_stream(reinterpret_cast<V2 *>(dst + (size_t)(RPT + 1) * Lxs), V2{z0.y, z1.y});
+                st_stream(reinterpret_cast<V2 *>(dst + (size_t)RPT * WT), V2{z0.x, z1.x});
+                st_stream(reinterpret_cast<V2 *>(dst + (size_t)(RPT + 1) * WT), V2{z0.y, z1.y});
             }
         }
     } else {
@@ -641,7 +773,7 @@ k_yfwd(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
                 v.x = L::real(lds, 2 * c2)[r];
                 v.y = L::real(lds, 2 * c2 + 1)[r];
             }
-            st_stream(reinterpret_cast<V2 *>(&W[(plane + R0 + r) * Lxs + col]), v);
+            st_stream(reinterpret_cast<V2 *>(&W[plane * Lxs + w_at<T>(R0 + r, col, Ly)]), v);
         }
     }
     }
@@ -660,6 +792,7 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
     constexpr int TP = TX / 2;  // column pairs: global accesses are 2 adjacent scalars
     constexpr int EPT = (TP * NR + NT - 1) / NT;
     using V2 = T __attribute__((ext_vector_type(2)));
+    constexpr int WT = w_tile<T>();  // W row step inside a column tile
     __shared__ __align__(16) T lds[L::SIZE];
     const int tid = threadIdx.x, l = tid / TPL, t = tid - l * TPL;
     const int nct = (Lx + TX - 1) / TX;
@@ -674,12 +807,12 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
         if constexpr (Y::RB) {
             int col = ct * TX + 2 * bc2;
             col = col < Lx ? col : Lx - 2;  // Lx is even
-            const T *src = &W[(plane + R0 + (size_t)RPT * bg) * Lxs + col];
+            const T *src = &W[plane * Lxs + w_at<T>(R0 + (size_t)RPT * bg, col, Ly)];
 #pragma unroll
-            for (int q = 0; q < RPT; ++q) pre[q] = ld_stream(reinterpret_cast<const V2 *>(src + (size_t)q * Lxs));
+            for (int q = 0; q < RPT; ++q) pre[q] = ld_stream(reinterpret_cast<const V2 *>(src + (size_t)q * WT));
             if (MODE == 2 && bg == Y::G - 1) {  // R2C: spectral rows N, N+1
-                pre_x[0] = ld_stream(reinterpret_cast<const V2 *>(src + (size_t)RPT * Lxs));
-                pre_x[1] = ld_stream(reinterpret_cast<const V2 *>(src + (size_t)(RPT + 1) * Lxs));
+                pre_x[0] = ld_stream(reinterpret_cast<const V2 *>(src + (size_t)RPT * WT));
+                pre_x[1] = ld_stream(reinterpret_cast<const V2 *>(src + (size_t)(RPT + 1) * WT));
             }
         } else {
 #pragma unroll
@@ -689,7 +822,7 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
                 const int r = e / TP, c2 = e - r * TP;
                 int col = ct * TX + 2 * c2;
                 col = col < Lx ? col : Lx - 2;  // Lx is even
-                pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[(plane + R0 + r) * Lxs + col]));
+                pre[q] = ld_stream(reinterpret_cast<const V2 *>(&W[plane * Lxs + w_at<T>(R0 + r, col, Ly)]));
             }
         }
     };
@@ -763,11 +896,11 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
                 v[2 * k] = V2{a0.x, a1.x} * scale;
                 v[2 * k + 1] = V2{a0.y, a1.y} * -scale;
             }
-            T *dst = &W[(plane + 1 + (size_t)RPT * bg) * Lxs + col];
+            T *dst = &W[plane * Lxs + w_at<T>(1 + (size_t)RPT * bg, col, Ly)];
 #pragma unroll
-            for (int q = 0; q < RPT; ++q) st_stream(reinterpret_cast<V2 *>(dst + (size_t)q * Lxs), v[q]);
-            if (bg == 0) st_stream(reinterpret_cast<V2 *>(&W[(plane + N + 1) * Lxs + col]), v[0]);
-            if (bg == Y::G - 1) st_stream(reinterpret_cast<V2 *>(&W[plane * Lxs + col]), v[RPT - 1]);
+            for (int q = 0; q < RPT; ++q) st_stream(reinterpret_cast<V2 *>(dst + (size_t)q * WT), v[q]);
+            if (bg == 0) st_stream(reinterpret_cast<V2 *>(&W[plane * Lxs + w_at<T>(N + 1, col, Ly)]), v[0]);
+            if (bg == Y::G - 1) st_stream(reinterpret_cast<V2 *>(&W[plane * Lxs + w_at<T>(0, col, Ly)]), v[RPT - 1]);
         }
     } else if constexpr (Y::RB) {
         // Makhoul order (as in k_yfwd), the signs of the DCT-III's conjugate
@@ -785,13 +918,13 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
                 v[4 * j + 3] = V2{b0.x, b1.x} * scale;
                 v[4 * j + 1] = V2{b0.y, b1.y} * -scale;
             }
-            T *dst = &W[(plane + 1 + (size_t)RPT * bg) * Lxs + col];
+            T *dst = &W[plane * Lxs + w_at<T>(1 + (size_t)RPT * bg, col, Ly)];
 #pragma unroll
-            for (int q = 0; q < RPT; ++q) st_stream(reinterpret_cast<V2 *>(dst + (size_t)q * Lxs), v[q]);
+            for (int q = 0; q < RPT; ++q) st_stream(reinterpret_cast<V2 *>(dst + (size_t)q * WT), v[q]);
             // wall ghost rows (fft_3d_poisson.f90:809-852)
-            if (bg == 0) st_stream(reinterpret_cast<V2 *>(dst - Lxs), bca0 * v[0] - bcb0 * v[1]);
+            if (bg == 0) st_stream(reinterpret_cast<V2 *>(dst - WT), bca0 * v[0] - bcb0 * v[1]);
             if (bg == Y::G - 1)
-                st_stream(reinterpret_cast<V2 *>(dst + (size_t)RPT * Lxs), bca1 * v[RPT - 1] - bcb1 * v[RPT - 2]);
+                st_stream(reinterpret_cast<V2 *>(dst + (size_t)RPT * WT), bca1 * v[RPT - 1] - bcb1 * v[RPT - 2]);
         }
     } else
     for (int e = tid; e < TP * (N + 2); e += NT) {
@@ -820,7 +953,7 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
             out = MODE == 0 ? bca1 * sample(N - 1) - bcb1 * sample(N - 2) : sample(0);
         else
             out = sample(rp - 1);
-        st_stream(reinterpret_cast<V2 *>(&W[(plane + rp) * Lxs + col]), out);
+        st_stream(reinterpret_cast<V2 *>(&W[plane * Lxs + w_at<T>(rp, col, Ly)]), out);
     }
     }
 }
@@ -877,7 +1010,7 @@ k_zthomas(T *__restrict__ W, T *__restrict__ cs, ZArgs z, const double *__restri
     const int kx = (int)(col % z.Lx);
     const int jp = z.jlo + (int)(col / z.Lx);
     const size_t pstride = (size_t)z.Ly * z.Lxs;
-    T *w = W + (size_t)jp * z.Lxs + kx;      // plane 0 of this column
+    T *w = W + w_at<T>(jp, kx, z.Ly);      // plane 0 of this column
     T *c = cs + (size_t)(col);               // scratch, stride Lx*njr
     const size_t cstride = (size_t)z.Lx * z.njr;
     const double lx = z.lamx[kx], ly = z.lamy[jp];
@@ -985,13 +1118,15 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     CT *xe = rc + 3 * PS;           // [P][S]: partition end values x_{K-1}
     CT *ps = ra;                    // [P][S]: partial sums, once the reduction is done
     const int tid = threadIdx.x, s = tid % S, p = tid / S;
-    const int tiles = (z.Lx + S - 1) / S;
     const int blk = (int)blockIdx.x;
-    const int jp = z.jlo + blk / tiles;
-    const int kx = (blk % tiles) * S + s;
+    // rows fastest inside a column tile: consecutive workgroups take
+    // consecutive rows of one 128-byte column tile of W (adjacent bytes)
+    const int ct = blk / z.njr;
+    const int jp = z.jlo + (blk - ct * z.njr);
+    const int kx = ct * S + s;
     const bool active = kx < z.Lx;
     const size_t pstride = (size_t)z.Ly * z.Lxs;
-    T *w = W + (size_t)jp * z.Lxs + (active ? kx : 0);
+    T *w = W + w_at<T>(jp, active ? kx : 0, z.Ly);
     const CT lx = active ? (CT)z.lamx[kx] : CT(0);
     const CT ly = (CT)z.lamy[jp];
     const bool pin = active && z.zx[kx] && z.zy[jp];
@@ -1195,8 +1330,7 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
         }
     }
     // mean of the solution from the (0,0) real column (:628-664)
-    const bool mean_tile = (jp == z.mean_jp) && (blk % tiles) * S <= z.mean_kx &&
-                           z.mean_kx < (blk % tiles) * S + S;
+    const bool mean_tile = (jp == z.mean_jp) && ct * S <= z.mean_kx && z.mean_kx < ct * S + S;
     if (mean_tile) {
         ps[p * S + s] = part;
         __syncthreads();
@@ -1264,10 +1398,10 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     double pre_lx, pre_ly;
     unsigned char pre_zx, pre_zy;
     auto fetch = [&](int tile) {
-        const int jp = z.jlo + tile / tiles_x;
-        int kx = (tile % tiles_x) * S + s;
+        const int jp = z.jlo + tile % z.njr;  // rows fastest inside a column tile
+        int kx = (tile / z.njr) * S + s;
         kx = kx < z.Lx ? kx : z.Lx - 1;
-        const T *w = W + (size_t)jp * z.Lxs + kx;
+        const T *w = W + w_at<T>(jp, kx, z.Ly);
 #pragma unroll
         for (int l = 0; l < K; ++l) {
             const int g = g0 + l;
@@ -1282,10 +1416,10 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     if (tile < ntiles) fetch(tile);
     __syncthreads();
     for (; tile < ntiles; tile += gridDim.x) {
-        const int jp = z.jlo + tile / tiles_x;
-        const int kx = (tile % tiles_x) * S + s;
+        const int jp = z.jlo + tile % z.njr;
+        const int kx = (tile / z.njr) * S + s;
         const bool active = kx < z.Lx;
-        T *w = W + (size_t)jp * z.Lxs + (active ? kx : 0);
+        T *w = W + w_at<T>(jp, active ? kx : 0, z.Ly);
         const double lx = active ? pre_lx : 0.0;
         const double ly = pre_ly;
         const bool pin = active && pre_zx && pre_zy;
@@ -1410,7 +1544,7 @@ k_zpipe(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
                 w[(size_t)(n3m + 1) * pstride] = (T)(bc[2] * xt - bc[3] * xb);
             }
         }
-        const int kx0 = (tile % tiles_x) * S;
+        const int kx0 = (tile / z.njr) * S;
         if (jp == z.mean_jp && kx0 <= z.mean_kx && z.mean_kx < kx0 + S) {
             ps[p * S + s] = part;
             __syncthreads();
@@ -1513,9 +1647,11 @@ k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m, con
             for (int t = 0; t < TY; ++t) zp[t] = P[pc + t * L1 + pl];
             const double hlo = P[pc + dlo], hhi = P[pc + dhi];
             const double am = xr.zam[k], ap = xr.zap[k];
-            // STORE: r of row j0+t goes to W32 row (k+1) Ly + j0+t+1
+            // STORE: r of row j0+t goes to W32 plane k+1, row j0+t+1 (rows of a
+            // column tile are one tile width apart)
             float *rrow = nullptr;
-            if constexpr (STORE) rrow = xr.rout + ((size_t)(k + 1) * (size_t)xr.Ly + (size_t)(j0 + 1)) * (size_t)xr.Lxs + i;
+            if constexpr (STORE)
+                rrow = xr.rout + (size_t)(k + 1) * (size_t)xr.Ly * (size_t)xr.Lxs + w_at<float>(j0 + 1, i, xr.Ly);
 #pragma unroll
             for (int t = 0; t < TY; ++t) {
                 const long q = pc + t * L1;
@@ -1531,7 +1667,7 @@ k_residual(const double *__restrict__ f, XRes xr, int n1m, int n2m, int n3m, con
                     sr += r * r;
                     sf += fv * fv;
                     sx += c[t];
-                    if constexpr (STORE) st_stream(rrow + (size_t)t * (size_t)xr.Lxs, (float)r);
+                    if constexpr (STORE) st_stream(rrow + (size_t)t * w_tile<float>(), (float)r);
                 }
             }
 #pragma unroll
