@@ -163,13 +163,16 @@ int ppf_plan_tune(ppf_plan *plan, const double *prhs, double *p, const double *d
  * makes later solves return PPF_ERR_UNSUPPORTED. */
 int ppf_plan_set_zvariant(ppf_plan *plan, int zvariant);
 
-/* The plan's pass variants: variants[0] x pass (0: one butterfly per thread
- * per stage, 1: half the threads with two, 2: as 0 without holding the next
- * tile in registers, 3: as 2 with one workgroup per tile instead of a
- * persistent grid), [1] y pass (0/1 as for x), [2] z shape (as
- * ppf_plan_set_zvariant). */
+/* The plan's pass variants: variants[0] x passes, [1] y pass (0/1 as for
+ * x), [2] z shape (as ppf_plan_set_zvariant).  The x forms are 0: one
+ * butterfly per thread per stage, 1: half the threads with two, 2: as 0
+ * without holding the next tile in registers, 3: as 2 with one workgroup per
+ * tile instead of a persistent grid.  variants[0] is f when both x passes use
+ * form f, and f + 16 i when the forward pass uses f and the inverse i (the
+ * tuner picks the two apart). */
 int ppf_plan_variants(const ppf_plan *plan, int variants[3]);
-/* Pins all three (x: 0..3, y: 0 or 1). */
+/* Pins all three (x: 0..3 for both x passes, or f + 16 i with f != i in
+ * 0..3; y: 0 or 1). */
 int ppf_plan_set_variants(ppf_plan *plan, int xvariant, int yvariant, int zvariant);
 
 /* ---- batched solves of independent problems ----------------------------- */

@@ -81,7 +81,8 @@ struct ppf_plan {
     int mean_kx, mean_jp;
     double mean_scale;
     int zvariant;          // z-pass kernel shape (0 = chosen by n3m)
-    int xvariant = 0;      // x-pass threads per line: 0 one butterfly per stage, 1 two
+    int xvariant = 0;      // x-forward pass variant (kXVariants forms)
+    int xvinv = 0;         // x-inverse pass variant (the tuner picks it apart)
     int yvariant = 0;      // y-pass threads per line: 0 one butterfly per stage, 1 two
     // device memory
     double *W = nullptr;   // spectral workspace (FP64 plans)
@@ -205,6 +206,15 @@ constexpr int kZVariants = 17;
 // prefetch, 3 the same kernel as 2 with one workgroup per tile (not
 // persistent); y-pass variants (YGeom): 0 and 1.
 constexpr int kXVariants = 4, kYVariants = 2;
+// The x value of ppf_plan_variants / ppf_plan_set_variants: f when both x
+// passes use form f, f + 16 i when the forward pass uses f and the inverse i.
+inline int x_variant_code(int f, int i) { return f == i ? f : f + 16 * i; }
+inline bool x_variant_decode(int v, int &f, int &i) {
+    if (v >= 0 && v < kXVariants) { f = i = v; return true; }
+    f = v % 16;
+    i = v / 16;
+    return v >= 16 && f < kXVariants && i < kXVariants && f != i;
+}
 
 // The z row table of a plan (filled by pass 1 when passed to dispatch_xfwd).
 inline ZRows z_rows(const ppf_plan *p, const double *dx3, const double *dmx3) {

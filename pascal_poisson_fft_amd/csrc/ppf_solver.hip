@@ -427,14 +427,14 @@ int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1
         std::lock_guard<std::mutex> lock(g_tune_mu);
         auto it = g_tuned.find(key);
         if (it != g_tuned.end()) {
-            p->xvariant = it->second[0];
+            (void)x_variant_decode(it->second[0], p->xvariant, p->xvinv);
             p->yvariant = it->second[1];
             const int rc = ppf_plan_set_zvariant(p, it->second[2]);
             if (zvariant) *zvariant = p->zvariant;
             return rc;
         }
     }
-    const int keep[3] = {p->xvariant, p->yvariant, p->zvariant};
+    const int keep[4] = {p->xvariant, p->yvariant, p->zvariant, p->xvinv};
     // per-stage minimum over 4 timed solves after 1 untimed (variants differ
     // by a few percent, about the run-to-run spread of one solve);
     // UNSUPPORTED if the current shapes do not fit the extents
@@ -451,23 +451,27 @@ int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1
         p->xvariant = keep[0];
         p->yvariant = keep[1];
         p->zvariant = keep[2];
+        p->xvinv = keep[3];
         return rc;
     };
-    // x and y passes: one run per variant pair times both passes (independent
-    // kernels); each pass keeps its faster variant
-    float bx = 0.f, by = 0.f;
-    int vx = 0, vy = 0;
+    // x and y passes: one run per variant times all four line passes
+    // (independent kernels); the x-forward, x-inverse and y passes each keep
+    // their faster variant
+    float bxf = 0.f, bxi = 0.f, by = 0.f;
+    int vxf = 0, vxi = 0, vy = 0;
     p->zvariant = keep[2];
     for (int v = 0; v < std::max(kXVariants, kYVariants); ++v) {
-        p->xvariant = v < kXVariants ? v : 0;
+        p->xvariant = p->xvinv = v < kXVariants ? v : 0;
         p->yvariant = v < kYVariants ? v : 0;
         float st[5];
         const int rc = time_stages(st);
         if (rc) return restore(rc);
-        if (v < kXVariants && (v == 0 || st[0] + st[4] < bx)) { bx = st[0] + st[4]; vx = v; }
+        if (v < kXVariants && (v == 0 || st[0] < bxf)) { bxf = st[0]; vxf = v; }
+        if (v < kXVariants && (v == 0 || st[4] < bxi)) { bxi = st[4]; vxi = v; }
         if (v < kYVariants && (v == 0 || st[1] + st[3] < by)) { by = st[1] + st[3]; vy = v; }
     }
-    p->xvariant = vx;
+    p->xvariant = vxf;
+    p->xvinv = vxi;
     p->yvariant = vy;
     // z pass: the partitioned (K,S) shapes and the pipelined wavefront forms
     static const int cand[] = {3, 1, 2, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17};
@@ -484,7 +488,7 @@ int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1
     p->zvariant = best ? best : keep[2];
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
-        g_tuned[key] = {p->xvariant, p->yvariant, p->zvariant};
+        g_tuned[key] = {x_variant_code(p->xvariant, p->xvinv), p->yvariant, p->zvariant};
     }
     if (zvariant) *zvariant = p->zvariant;
     return PPF_OK;
@@ -492,18 +496,20 @@ int ppf_plan_tune(ppf_plan *p, const double *prhs, double *P, const double *dmx1
 
 int ppf_plan_variants(const ppf_plan *p, int variants[3]) {
     if (!p || !variants) return PPF_ERR_INVALID_ARG;
-    variants[0] = p->xvariant;
+    variants[0] = x_variant_code(p->xvariant, p->xvinv);
     variants[1] = p->yvariant;
     variants[2] = p->zvariant;
     return PPF_OK;
 }
 
 int ppf_plan_set_variants(ppf_plan *p, int xvariant, int yvariant, int zvariant) {
-    if (!p || xvariant < 0 || xvariant >= kXVariants || yvariant < 0 || yvariant >= kYVariants)
+    int xf = 0, xi = 0;
+    if (!p || !x_variant_decode(xvariant, xf, xi) || yvariant < 0 || yvariant >= kYVariants)
         return PPF_ERR_INVALID_ARG;
     const int rc = ppf_plan_set_zvariant(p, zvariant);
     if (rc) return rc;
-    p->xvariant = xvariant;
+    p->xvariant = xf;
+    p->xvinv = xi;
     p->yvariant = yvariant;
     return PPF_OK;
 }

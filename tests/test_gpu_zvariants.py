@@ -133,6 +133,35 @@ def test_xy_variants_match_oracle(pbc, n1, n2, n3):
         plan.clean()
 
 
+@pytest.mark.parametrize("pbc", [(False, False), (True, False)], ids=["NN", "PN"])
+def test_split_x_variants_match_oracle(pbc):
+    """The x-forward and x-inverse passes on different forms (variants value
+    f + 16 i: forward f, inverse i), as the tuner may pick them, against the
+    oracle at 1e-12; the value reads back as set, and a pair that is not a
+    valid split is rejected."""
+    n1, n2, n3 = 512, 32, 8
+    L, ax, ay, az, f, ref = _case(n1, n2, n3, pbc, seed=11)
+    plan = api.FFTPoisson(n1, n2, n3, L, (pbc[0], pbc[1], False))
+    dev = api.Problem3D(f, ax, ay, az)
+    try:
+        for xf, xi in ((2, 3), (3, 2), (0, 3), (3, 0)):
+            code = xf + 16 * xi
+            print(f"{n1}x{n2}x{n3} pbc={pbc} x forward {xf} inverse {xi}", flush=True)
+            plan.set_variants(code, 0, 0)
+            assert plan.variants() == (code, 0, 0)
+            dev.p.fill_zero()
+            plan.solve(dev.rhs, dev.p, dev.dmx1, dev.dmx2, dev.dmx3, dev.dx3)
+            api.sync()
+            err = float(np.linalg.norm(dev.p.download() - ref) / np.linalg.norm(ref))
+            assert err <= 1e-12, (xf, xi, err)
+        for bad in (4, 1 + 16, 5 + 16, 2 + 16 * 4, -1):
+            with pytest.raises(api.PPFError):
+                plan.set_variants(bad, 0, 0)
+    finally:
+        dev.free()
+        plan.clean()
+
+
 @pytest.mark.parametrize("xv", [2, 3])
 @pytest.mark.parametrize("pbc", [(False, False), (True, False)], ids=["NN", "PN"])
 def test_x_variant_without_prefetch_fp32_ir(pbc, xv):
@@ -167,7 +196,8 @@ def test_tuner_caches_per_shape_and_keeps_parity():
         z = plans[0].tune(*args)
         v = plans[0].variants()
         print(f"tuned variants {v}", flush=True)
-        assert v[2] == z and 0 <= v[0] < 4 and v[1] in (0, 1) and z > 0
+        xf, xi = (v[0], v[0]) if v[0] < 4 else (v[0] % 16, v[0] // 16)
+        assert v[2] == z and 0 <= xf < 4 and 0 <= xi < 4 and v[1] in (0, 1) and z > 0
         assert plans[1].tune(*args) == z and plans[1].variants() == v
         for pl in plans:
             dev.p.fill_zero()
