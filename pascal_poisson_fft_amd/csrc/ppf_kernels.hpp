@@ -216,7 +216,7 @@ struct XRes {
     double ih1, ih2;          // 1/h^2
     int pbc1, pbc2;
     float *rout;              // nullptr, or W32: r stored in FP32 where pass 1 reads line (j,k)
-    int Ly, Lxs;              // W32 row count per plane and row stride
+    int Ly, Lxs;              // W32 rows per plane and scalars per row (plane = Ly * Lxs)
 };
 
 // Sum of v over a workgroup of NT <= 256 threads (any count); thread 0 gets
@@ -260,7 +260,7 @@ k_xfwd(const double *__restrict__ in, T *W, int nlines, int n2m, int Ly,
     using L = typename X::L;
     constexpr int M = G::M, TPL = X::TPL, LPB = X::LPB, NT = X::NT;
     constexpr int LX = PER ? N + 2 : N;
-    constexpr int LXS = w_stride(LX, (int)sizeof(T));  // W row stride
+    constexpr int LXS = w_stride(LX, (int)sizeof(T));  // scalars per W row (a plane is Ly * LXS)
     constexpr int H = N / 2, HX = LX / 2;          // sample pairs per line / per W row
     // W-side element order: PT pairs fill one 128-byte row of a column tile;
     // consecutive lanes take the PT pairs of one line, then the same pairs
@@ -433,7 +433,7 @@ k_xinv(const T *__restrict__ W, double *__restrict__ P, int nlines, int Ly,
     using L = typename X::L;
     constexpr int M = G::M, TPL = X::TPL, LPB = X::LPB, NT = X::NT;
     constexpr int LX = PER ? N + 2 : N;
-    constexpr int LXS = w_stride(LX, (int)sizeof(T));  // W row stride
+    constexpr int LXS = w_stride(LX, (int)sizeof(T));  // scalars per W row (a plane is Ly * LXS)
     constexpr int LP = N + 2;
     constexpr int HX = LX / 2, HP = LP / 2;  // scalar pairs per W row / per P row
     // W rows are read in column-tile order (PT pairs of one row's tile, then
@@ -969,7 +969,7 @@ k_yinv(T *__restrict__ W, int Lx, int Lxs, int Ly, int nplanes, const cx<T> *__r
 // the (0,0) real column stores the solution mean for pass 5 (:628-664).
 // ---------------------------------------------------------------------------
 struct ZArgs {
-    int Lx, Lxs, Ly, n3m, jlo, njr;   // Lx columns per row, Lxs the row stride
+    int Lx, Lxs, Ly, n3m, jlo, njr;   // Lx columns per row, Lxs scalars per row (a plane is Ly * Lxs)
     const double *lamx, *lamy;        // per scalar column / per row
     const unsigned char *zx, *zy;     // zero-mode flags
     int mean_kx, mean_jp;             // real (0,0) column

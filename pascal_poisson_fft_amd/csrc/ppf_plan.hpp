@@ -74,7 +74,7 @@ unsigned persistent_grid(K kernel, int nthreads, int ntiles, size_t dyn_lds = 0)
 struct ppf_plan {
     ppf_desc d;
     int Lx, Ly, Lz;        // workspace extents (scalars)
-    int Lxs;               // workspace row stride (Lx rounded up to 128 bytes)
+    int Lxs;               // workspace scalars per row: Lx rounded up to 128 bytes (whole column tiles)
     int xkind, ykind;      // ppf::AxisKind
     int ymode;             // y pass MODE (0 DCT, 1 C2C, 2 R2C)
     int zjlo, znjr;        // rows solved by the z pass
