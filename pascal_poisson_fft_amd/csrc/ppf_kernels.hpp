@@ -1126,7 +1126,9 @@ k_zpart(T *__restrict__ W, ZArgs z, const double *__restrict__ zam, const double
     const int kx = ct * S + s;
     const bool active = kx < z.Lx;
     const size_t pstride = (size_t)z.Ly * z.Lxs;
-    T *w = W + w_at<T>(jp, active ? kx : 0, z.Ly);
+    // columns past Lx (the last tile's padding) load the last real column,
+    // which lies in the same 128-byte rows, and are masked below
+    T *w = W + w_at<T>(jp, active ? kx : z.Lx - 1, z.Ly);
     const CT lx = active ? (CT)z.lamx[kx] : CT(0);
     const CT ly = (CT)z.lamy[jp];
     const bool pin = active && z.zx[kx] && z.zy[jp];
