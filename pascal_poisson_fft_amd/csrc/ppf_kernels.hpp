@@ -153,11 +153,15 @@ struct XGeom {
     static constexpr int TPL = (G::TPL >> (XV == 1 ? 1 : 0)) > 0 ? (G::TPL >> (XV == 1 ? 1 : 0)) : 1;
     static constexpr int LPB = (256 / TPL) > 0 ? (256 / TPL) : 1;
     static constexpr int NT = LPB * TPL;
-    // the bank pad where the padded tile still fits the CU's LDS
+    // the bank pad where a thread keeps one line per tile (the longer lines:
+    // from 128 points in FP64, 256 in FP32); on shorter lines the larger tile
+    // cost more occupancy than the pad gained (96-problem batch 5.57-5.60
+    // against 5.60-5.64 ms)
     static constexpr int PADX = x_line_pad<T>(G::STRIDE);
-    static constexpr bool PAD_FITS =
+    static constexpr bool PAD_USE =
+        NT % ((w_tile<T>() / 2) * LPB) == 0 &&
         LPB * (2 * lds_slots(G::STRIDE) + PADX) + 2 * lds_table_slots<G::M, !PER>() <= (160 * 1024) / (int)sizeof(T);
-    using L = LdsLines<T, G::M, G::STRIDE, LPB, !PER, PAD_FITS ? PADX : PPF_LINE_PAD>;
+    using L = LdsLines<T, G::M, G::STRIDE, LPB, !PER, PAD_USE ? PADX : PPF_LINE_PAD>;
     // register target: PPF_XWPE waves per SIMD, or fewer where the LDS tile
     // (very short or 1024-point lines) admits fewer workgroups per CU
     static constexpr int WG_LDS = (160 * 1024) / (L::SIZE * (int)sizeof(T));
